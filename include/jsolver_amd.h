@@ -1,0 +1,180 @@
+/*
+ * jsolver_amd C ABI: geometric multigrid for  div(D grad phi) - lambda phi = rhs
+ * on ghost-padded (m+3)-per-axis grids, as hand-written HIP kernels for
+ * MI355X (gfx950).
+ *
+ * The reference (davef96/jsolver) is a pure Python/JAX package with no FFI;
+ * each entry point below replaces one logical operation of its solver path,
+ * cited as file:line into the reference.  The Python package jsolver_amd
+ * binds these through ctypes (jsolver_amd/_lib.py) behind the reference's own
+ * function names (solve_2d, solve_3d, do_BCs, grid_1D).
+ *
+ * Conventions
+ *   - Every function returns 0 on success, otherwise a nonzero status; the
+ *     message is available from jsv_last_error() (thread-local).
+ *   - Array pointers are device pointers to C-contiguous row-major arrays in
+ *     the reference's layout: 2D (my+3, mx+3), 3D (mz+3, my+3, mx+3).
+ *     Inputs are never modified; the caller owns them.
+ *   - A plan owns its per-level device buffers (allocated once, at creation)
+ *     and is bound to no stream: every call takes the stream it runs on.  A
+ *     plan is not thread-safe; calls on one plan must not overlap.
+ *   - "stream" is a hipStream_t passed as void* (NULL = default stream).
+ */
+#ifndef JSOLVER_AMD_H
+#define JSOLVER_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jsv_plan jsv_plan;
+
+/* precision of the multigrid cycles */
+#define JSV_FP64 0   /* every cycle in FP64 (the reference's arithmetic) */
+#define JSV_MIXED 1  /* FP32 cycles inside FP64 iterative refinement */
+
+/* stop rules */
+#define JSV_STOP_DISTANCE 0     /* max|phi_old - phi| <= epsilon (solver_2d.py:465-477) */
+#define JSV_STOP_REL_RESIDUAL 1 /* ||rhs - L phi||_2 / ||rhs||_2 <= rtol on the unknowns */
+
+typedef struct jsv_config {
+    int fixed;              /* 1: exactly max_iters cycles (solver_2d.py:461-463) */
+    double epsilon;         /* distance threshold, default 1e-12 */
+    int gamma;              /* 1 V-cycle, 2 W-cycle */
+    int fcycle;             /* F-cycle (solver_2d.py:609-611) */
+    int skip_pre;           /* skip level-0 pre-smoothing after the first cycle */
+    int nu1, nu2;           /* pre/post smoothing sweeps */
+    int64_t max_iters;      /* max outer cycles (int32 max = unbounded) */
+    int stop;               /* JSV_STOP_* (dynamic mode only) */
+    double rtol;            /* relative-residual threshold for JSV_STOP_REL_RESIDUAL */
+    int inner_cycles;       /* JSV_MIXED: FP32 cycles per FP64 refinement step */
+    int check_every;        /* dynamic mode: cycles launched between host checks (>=1) */
+    double *history;        /* optional host array: per-cycle distance or relative residual */
+    int64_t history_len;    /* capacity of history */
+} jsv_config;
+
+typedef struct jsv_result {
+    int64_t count;          /* cycles performed (FP32 cycles in JSV_MIXED) */
+    int64_t outer_steps;    /* FP64 refinement steps (JSV_MIXED), else = count */
+    double distance;        /* last max|phi_old - phi| (JSV_STOP_DISTANCE) */
+    double rel_residual;    /* last ||rhs - L phi|| / ||rhs|| when computed, else -1 */
+    int nonfinite;          /* 1 if a NaN/Inf was seen in the stop quantity */
+} jsv_result;
+
+int jsv_abi_version(void);
+const char *jsv_last_error(void);
+
+/* Plan creation: level layout and grid spacing.
+ * Replaces Solver_2D.__init__/compute_shapes/compute_idel (solver_2d.py:247-324)
+ * and Solver_3D's (solver_3d.py:248-336).  dim = 2 ignores mz/idel_z.
+ * Errors (nonzero status): resolution not a power of 2, coarsest level > 5. */
+int jsv_plan_create(int dim, int mx, int my, int mz, double idel_x, double idel_y,
+                    double idel_z, int precision, jsv_plan **out);
+/* Batched 2D plan: `batch` independent problems on the same grid, stored as
+ * consecutive (my+3, mx+3) arrays (array shape (batch, my+3, mx+3)) and solved
+ * together, each with its own coefficients, right-hand side and stop rule.
+ * No reference counterpart (the reference solves one problem per call);
+ * 3D requires batch = 1. */
+int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double idel_x,
+                            double idel_y, double idel_z, int precision, jsv_plan **out);
+int jsv_plan_destroy(jsv_plan *plan);
+int jsv_plan_levels(const jsv_plan *plan, int *levels);
+/* shape of level l in array-axis order (nz, ny, nx); nz = 1 in 2D */
+int jsv_plan_level_shape(const jsv_plan *plan, int level, int shape[3]);
+/* bytes of device memory the plan holds */
+int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
+/* Working-set bound of one z-slab in the 3D smoothing schedule (default 96 MiB;
+ * 0 = whole level per launch).  Results do not depend on it.  No reference
+ * counterpart (a launch-schedule knob). */
+int jsv_plan_set_slab_bytes(jsv_plan *plan, int64_t bytes);
+
+/* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /
+ * compute_discretization (solver_2d.py:326-421, solver_3d.py:338-429).
+ * lam / dxx / dyy / dzz / dxy: device arrays of the fine shape, or NULL to use
+ * the scalar (lam_scalar, d_scalar[0..2] = D_xx, D_yy, D_zz).  D is either all
+ * scalar or all arrays (solver_2d.py:444-447).  dxy (2D only) requires arrays. */
+int jsv_plan_set_operator(jsv_plan *plan, const double *lam, double lam_scalar,
+                          const double *dxx, const double *dyy, const double *dzz,
+                          const double d_scalar[3], const double *dxy, void *stream);
+
+/* Whole solve.  Replaces Solver_2D.solve / Solver_3D.solve (solver_2d.py:423-479,
+ * solver_3d.py:431-486).  phi_out may equal phi_in.  Synchronises the stream
+ * in dynamic mode (it reads the stop quantity back) and at the end. */
+int jsv_solve(jsv_plan *plan, const double *phi_in, const double *rhs, double *phi_out,
+              const jsv_config *cfg, jsv_result *res, void *stream);
+/* Batched solve: counts (host int64[batch], may be NULL) receives each
+ * problem's cycle count, stopvals (host double[batch], may be NULL) its last
+ * stop quantity; res summarises (count = max over the batch).  A problem that
+ * has met its stop rule is frozen while the others continue, so every
+ * problem's result equals its unbatched solve. */
+int jsv_solve_batched(jsv_plan *plan, const double *phi_in, const double *rhs, double *phi_out,
+                      const jsv_config *cfg, jsv_result *res, int64_t *counts, double *stopvals,
+                      void *stream);
+
+/* ---- single operations on the plan's level buffers (parity tests, adjoint) --
+ * prec selects the hierarchy: JSV_FP64 (double buffers) or JSV_MIXED (float). */
+#define JSV_BUF_PHI 0
+#define JSV_BUF_RHS 1
+/* copy a caller device array (double for JSV_FP64, float for JSV_MIXED) into /
+ * out of a level buffer */
+int jsv_level_upload(jsv_plan *plan, int prec, int level, int which, const void *src, void *stream);
+int jsv_level_download(jsv_plan *plan, int prec, int level, int which, void *dst, void *stream);
+/* sweeps x red_black_gauss_seidel on a level (solver_2d.py:670-692, solver_3d.py:679-706) */
+int jsv_smooth(jsv_plan *plan, int prec, int level, int sweeps, void *stream);
+/* rhs[level+1] = restrict(rhs - L phi) on level (defect, solver_2d.py:616-647,
+ * solver_3d.py:622-652) */
+int jsv_restrict(jsv_plan *plan, int prec, int level, void *stream);
+/* phi[level] += prolongate(phi[level+1]) (solver_2d.py:612, 649-668;
+ * solver_3d.py:618, 654-677); nu2 selects the skipped interpolations */
+int jsv_prolong_add(jsv_plan *plan, int prec, int level, int nu2, void *stream);
+/* one multigrid cycle on the level-0 buffers (multigrid_cycle, solver_2d.py:582-614),
+ * including the leading do_BCs(phi) */
+int jsv_cycle(jsv_plan *plan, int prec, const jsv_config *cfg, int skip_pre, void *stream);
+
+/* ---- reductions and residuals (no live counterpart in the reference; basis:
+ * the dead compute_residual/get_l2_residual, solver_2d.py:496-545, with the
+ * correct z term instead of solver_3d.py:517's y coefficients) ---- */
+/* out_r (may be NULL): rhs - L phi on the unknowns, zero elsewhere (FP64);
+ * phi's boundary nodes and ghosts are taken as the zero Dirichlet values.
+ * norm2_out (host, may be NULL): ||rhs - L phi||_2^2 over the unknowns, one
+ * value per batched problem. */
+int jsv_residual(jsv_plan *plan, const double *phi, const double *rhs, double *out_r,
+                 double *norm2_out, void *stream);
+/* max |a - b| over n elements (distance, solver_2d.py:481-494); host result */
+int jsv_max_abs_diff(const double *a, const double *b, int64_t n, double *out, void *stream);
+
+/* ---- boundary handler (do_BCs, boundary_handler_2d.py:3-29 /
+ * boundary_handler_3d.py:3-35), in place on a caller array.  vals_* may be
+ * NULL (zero).  3D: like the reference, the z faces take vals_yb/vals_ye. ---- */
+int jsv_do_bcs_2d(double *arr, int ny, int nx, int rim, int direction,
+                  const double *vals_xb, const double *vals_xe,
+                  const double *vals_yb, const double *vals_ye, void *stream);
+int jsv_do_bcs_3d(double *arr, int nz, int ny, int nx, int rim, int direction,
+                  const double *vals_xb, const double *vals_xe,
+                  const double *vals_yb, const double *vals_ye, void *stream);
+
+/* ---- reverse pass of fixed-mode solves (FP64, 5/7-point operators) ----
+ * Replaces the reference's jax.vjp over a fixed solve
+ * (examples/benchmark.py:1184-1226; tests/test_solver_2d.py:200-207).
+ * jsv_forward_checkpoints runs cfg->max_iters cycles from phi0 and stores the
+ * iterate entering every cycle and the final one: ckpt holds
+ * (max_iters + 1) x n doubles (n = fine array size).  jsv_adjoint takes the
+ * cotangent g_out of the final phi and returns the cotangents of phi0, rhs,
+ * lambda and the D components (any output may be NULL); each cycle's interior
+ * states are recomputed from ckpt, not stored.  The scalar-D / scalar-lambda
+ * cases return the per-cell cotangent of the broadcast array (sum it for the
+ * scalar).  D_xy is not supported. */
+int jsv_forward_checkpoints(jsv_plan *plan, const double *phi0, const double *rhs,
+                            const jsv_config *cfg, double *ckpt, void *stream);
+int jsv_adjoint(jsv_plan *plan, const double *phi0, const double *rhs, const double *ckpt,
+                const jsv_config *cfg, const double *g_out, double *g_phi0, double *g_rhs,
+                double *g_lam, double *g_dxx, double *g_dyy, double *g_dzz, void *stream);
+/* device bytes held by the plan's reverse-pass buffers (tape included) */
+int jsv_adjoint_device_bytes(const jsv_plan *plan, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JSOLVER_AMD_H */

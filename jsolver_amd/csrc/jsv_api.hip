@@ -1,0 +1,1362 @@
+// Host side of the C ABI declared in include/jsolver_amd.h: plans (per-level
+// device buffers), coefficient setup, the recursive V/W/F cycle and the solve
+// loop.  The reference builds the same recursion at trace time in Python
+// (multigrid_cycle, jsolver/solver_2d.py:582-614); here it is a C++ recursion
+// that enqueues kernels on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/jsolver_amd.h"
+#include "jsv_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+using jsv::Geo;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return 1;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                         \
+    } while (0)
+
+#define LAUNCHCHK()                                                                           \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            return fail("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                            \
+    } while (0)
+
+#define RETCHK(expr)          \
+    do {                      \
+        int r_ = (expr);      \
+        if (r_ != 0) return r_; \
+    } while (0)
+
+template <typename T>
+struct Hier {
+    std::vector<Geo> g;
+    std::vector<T*> phi, rhs, cx, cy, cz, cc, drr, dlr;
+};
+
+inline dim3 grid_points(const Geo& g, int dim, int xcount)
+{
+    // xcount points along x starting at 2, my-1 rows, mz-1 planes (3D) or
+    // one z per batched problem (2D)
+    return dim3((unsigned)((xcount + jsv::BX - 1) / jsv::BX),
+                (unsigned)((g.my - 1 + jsv::BY - 1) / jsv::BY),
+                (unsigned)(dim == 3 ? g.mz - 1 : g.nz));
+}
+// one thread per element, for kernels without a grid-stride loop
+inline dim3 grid_exact(long n)
+{
+    long b = (n + 255) / 256;
+    return dim3((unsigned)(b < 1 ? 1 : b));
+}
+// capped grid for grid-stride kernels
+inline dim3 grid_flat(long n)
+{
+    long b = (n + 255) / 256;
+    if (b > 65535L * 16) b = 65535L * 16;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+}  // namespace
+
+struct jsv_plan {
+    int dim = 0, prec = 0, levels = 0;
+    int batch = 1;                  // 2D: independent problems solved together
+    int m[3] = {0, 0, 0};          // x, y, z resolutions of level 0
+    double idel[3] = {0, 0, 0};
+    bool op_set = false, spatial = false, offdiag = false;
+    Hier<double> h64;
+    Hier<float> h32;
+    double* phi_old = nullptr;      // level-0 copy for the distance
+    double* partials = nullptr;
+    long npartials = 0;
+    double* dscal = nullptr;        // device scalars: 4 slots x batch
+    double* hscal = nullptr;        // pinned host mirror
+    int* d_act = nullptr;           // per-problem "still iterating" flags
+    int* h_act = nullptr;
+    long slab_bytes = 96L << 20;    // slab size of the 3D smoothing schedule (0: whole level)
+    bool use_act = false;           // kernels consult d_act (dynamic batched solves)
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    const int* act_ptr() const { return use_act ? d_act : nullptr; }
+};
+
+namespace {
+
+int dalloc(jsv_plan* P, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) return fail("hipMemset failed: %s", hipGetErrorString(e));
+    P->allocs.push_back(*p);
+    P->bytes += (int64_t)bytes;
+    return 0;
+}
+
+template <typename T>
+int alloc_hier(jsv_plan* P, Hier<T>& H, bool with_coeffs)
+{
+    H.g.clear();
+    const int L = P->levels;
+    int lv[3];
+    for (int a = 0; a < P->dim; ++a) lv[a] = (int)std::lround(std::log2((double)P->m[a])) + 1;
+    for (int l = 0; l < L; ++l) {
+        Geo g{};
+        g.mx = 1 << (lv[0] - 1 - l);
+        g.my = 1 << (lv[1] - 1 - l);
+        g.mz = (P->dim == 3) ? 1 << (lv[2] - 1 - l) : 0;
+        g.nx = g.mx + 3;
+        g.ny = g.my + 3;
+        g.nz = (P->dim == 3) ? g.mz + 3 : P->batch;
+        g.sy = g.nx;
+        g.sz = (long)g.nx * g.ny;
+        g.n = g.sz * g.nz;
+        H.g.push_back(g);
+    }
+    auto one = [&](std::vector<T*>& v, bool need) -> int {
+        v.assign(L, nullptr);
+        if (!need) return 0;
+        for (int l = 0; l < L; ++l) RETCHK(dalloc(P, (void**)&v[l], sizeof(T) * H.g[l].n));
+        return 0;
+    };
+    RETCHK(one(H.phi, true));
+    RETCHK(one(H.rhs, true));
+    RETCHK(one(H.cx, with_coeffs));
+    RETCHK(one(H.cy, with_coeffs));
+    RETCHK(one(H.cz, with_coeffs && P->dim == 3));
+    RETCHK(one(H.cc, with_coeffs));
+    RETCHK(one(H.drr, false));
+    RETCHK(one(H.dlr, false));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// elementary operations
+// ---------------------------------------------------------------------------
+template <typename T>
+int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+{
+    const Geo& g = H.g[l];
+    if (P->dim == 2) {
+        if (P->offdiag) {
+            // the reference's sub-colour order (0,0),(1,1),(0,1),(1,0)
+            static const int order[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+            for (int k = 0; k < 4; ++k) {
+                const int lx = order[k][0], ly = order[k][1];
+                dim3 gr((unsigned)((g.mx / 2 + jsv::BX - 1) / jsv::BX),
+                        (unsigned)((g.my / 2 + jsv::BY - 1) / jsv::BY), 1);
+                gr.z = (unsigned)g.nz;
+                hipLaunchKernelGGL(jsv::k_rb2_diag<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
+                                   H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cc[l], H.drr[l],
+                                   H.dlr[l], g, lx, ly, P->act_ptr());
+            }
+        } else {
+            dim3 gr = grid_points(g, 2, g.mx / 2);
+            hipLaunchKernelGGL(jsv::k_rb2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
+                               H.rhs[l], H.cx[l], H.cy[l], H.cc[l], g, 0, P->act_ptr());
+            hipLaunchKernelGGL(jsv::k_rb2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
+                               H.rhs[l], H.cx[l], H.cy[l], H.cc[l], g, 1, P->act_ptr());
+        }
+    } else {
+        // Slab schedule: red on z-slab k, then black on slab k-1, so the black
+        // half re-reads a slab the red half fetched two launches earlier (still
+        // in the Infinity Cache) instead of streaming the level twice from HBM.
+        // Each point sees exactly the values of the plain red-then-black order:
+        // red on slab k reads black planes not yet updated, black on slab k-1
+        // reads red planes that are all updated.
+        const int planes = g.mz - 1;               // unknown planes z = 2..mz
+        const long plane_bytes = (long)g.sz * sizeof(T) * 6;
+        int slab = planes;
+        if (P->slab_bytes > 0 && plane_bytes * planes > P->slab_bytes * 2) {
+            slab = (int)(P->slab_bytes / plane_bytes);
+            if (slab < 2) slab = 2;
+        }
+        dim3 gr = grid_points(g, 3, g.mx / 2);
+        auto launch = [&](int par, int za, int zb) {
+            dim3 gs(gr.x, gr.y, (unsigned)(zb - za));
+            hipLaunchKernelGGL(jsv::k_rb3<T>, gs, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
+                               H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, par, za);
+        };
+        const int zend = g.mz + 1;
+        int prev_a = -1, prev_b = -1;
+        for (int za = 2; za < zend; za += slab) {
+            const int zb = za + slab < zend ? za + slab : zend;
+            launch(1, za, zb);
+            if (prev_a >= 0) launch(0, prev_a, prev_b);
+            prev_a = za;
+            prev_b = zb;
+        }
+        launch(0, prev_a, prev_b);
+    }
+    return 0;
+}
+
+template <typename T>
+int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+{
+    const Geo& gf = H.g[l];
+    const Geo& gc = H.g[l + 1];
+    dim3 gr = grid_points(gc, P->dim, gc.mx - 1);
+    if (P->dim == 2)
+        hipLaunchKernelGGL(jsv::k_restrict2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
+                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cc[l],
+                           P->offdiag ? H.drr[l] : nullptr, P->offdiag ? H.dlr[l] : nullptr, gf, gc,
+                           P->act_ptr());
+    else
+        hipLaunchKernelGGL(jsv::k_restrict3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
+                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, gc);
+    return 0;
+}
+
+template <typename T>
+int prolong(const jsv_plan* P, Hier<T>& H, int l, int nu2, hipStream_t s)
+{
+    const Geo& gf = H.g[l];
+    const Geo& gc = H.g[l + 1];
+    dim3 gr = grid_points(gf, P->dim, gf.mx - 1);
+    if (P->dim == 2) {
+        const int full = (nu2 <= 0 || P->offdiag) ? 1 : 0;
+        hipLaunchKernelGGL(jsv::k_prolong2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
+                           H.phi[l + 1], gf, gc, full, P->act_ptr());
+    } else {
+        hipLaunchKernelGGL(jsv::k_prolong3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
+                           H.phi[l + 1], gf, gc, nu2 <= 0 ? 1 : 0);
+    }
+    return 0;
+}
+
+template <typename T>
+int zero(T* a, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(jsv::k_fill<T>, grid_flat(n), dim3(256), 0, s, a, n, (T)0);
+    return 0;
+}
+
+// multigrid_cycle (solver_2d.py:582-614 / solver_3d.py:588-620)
+template <typename T>
+int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcyc, bool skip_pre,
+             hipStream_t s)
+{
+    if (l == P->levels - 2) {
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) RETCHK(smooth(P, H, l, s));
+        return 0;
+    }
+    if (!skip_pre)
+        for (int i = 0; i < c.nu1; ++i) RETCHK(smooth(P, H, l, s));
+    RETCHK(restrict_(P, H, l, s));
+    RETCHK(zero(H.phi[l + 1], H.g[l + 1].n, s));
+    if (fcyc)
+        for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, true, false, s));
+    for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, false, false, s));
+    RETCHK(prolong(P, H, l, c.nu2, s));
+    for (int i = 0; i < c.nu2; ++i) RETCHK(smooth(P, H, l, s));
+    return 0;
+}
+
+int zero_rim_level0(jsv_plan* P, double* a, hipStream_t s)
+{
+    const Geo& g = P->h64.g[0];
+    for (int ph = 0; ph < P->dim; ++ph)
+        hipLaunchKernelGGL(jsv::k_bcs, grid_exact(g.n), dim3(256), 0, s, a, g, P->dim, 1, -1,
+                           nullptr, nullptr, nullptr, nullptr, ph);
+    return 0;
+}
+
+// Per-problem scalar slots: slot k of problem b lives at dscal[k * batch + b].
+// 3D levels are one problem; 2D levels hold `batch` problems along z.
+inline int nseg(const jsv_plan* P) { return P->dim == 3 ? 1 : P->batch; }
+
+// ||a||^2 over the unknowns of each problem of the fine level -> slot
+template <typename T>
+int norm2_unknowns(jsv_plan* P, const T* a, int slot, hipStream_t s)
+{
+    const Geo& g = P->h64.g[0];
+    dim3 gr = grid_points(g, P->dim, g.mx - 1);
+    hipLaunchKernelGGL(jsv::k_norm2_unknowns<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, a, g,
+                       P->partials, P->dim);
+    const long per = (long)gr.x * gr.y * gr.z / nseg(P);
+    hipLaunchKernelGGL(jsv::k_sum_partials, dim3((unsigned)nseg(P)), dim3(1024), 0, s, P->partials,
+                       per, P->dscal + (long)slot * P->batch);
+    return 0;
+}
+
+// FP64 residual of the fine level; optional store (double or float) and
+// ||r||^2 per problem -> slot
+template <typename TO>
+int residual64(jsv_plan* P, const double* phi, const double* rhs, TO* rout, int slot, hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    const Geo& g = H.g[0];
+    dim3 gr = grid_points(g, P->dim, g.mx - 1);
+    hipLaunchKernelGGL((jsv::k_residual<double, double, TO>), gr, dim3(jsv::BX, jsv::BY), 0, s,
+                       phi, rhs, H.cx[0], H.cy[0], P->dim == 3 ? H.cz[0] : nullptr, H.cc[0],
+                       P->offdiag ? H.drr[0] : nullptr, P->offdiag ? H.dlr[0] : nullptr, g, rout,
+                       P->partials, P->dim);
+    const long per = (long)gr.x * gr.y * gr.z / nseg(P);
+    hipLaunchKernelGGL(jsv::k_sum_partials, dim3((unsigned)nseg(P)), dim3(1024), 0, s, P->partials,
+                       per, P->dscal + (long)slot * P->batch);
+    return 0;
+}
+
+// max|a-b| over each problem's whole array -> slot
+int maxdiff(jsv_plan* P, const double* a, const double* b, int slot, hipStream_t s)
+{
+    const int ns = nseg(P);
+    const long n = P->h64.g[0].n / ns;
+    unsigned long long* out = reinterpret_cast<unsigned long long*>(P->dscal + (long)slot * P->batch);
+    hipLaunchKernelGGL(jsv::k_fill<unsigned long long>, dim3(1), dim3(256), 0, s, out, (long)ns, 0ull);
+    long nb = (n + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(jsv::k_maxabsdiff<double>, dim3((unsigned)nb, (unsigned)ns), dim3(256), 0, s,
+                       a, b, n, out);
+    return 0;
+}
+
+// copy slots [0, nslots) of every problem to the host and wait
+int read_scalars(jsv_plan* P, int nslots, hipStream_t s)
+{
+    HIPCHK(hipMemcpyAsync(P->hscal, P->dscal, sizeof(double) * nslots * P->batch,
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int check_cfg(const jsv_config* c)
+{
+    if (c == nullptr) return fail("config is NULL");
+    if (c->gamma < 1) return fail("gamma must be >= 1 (got %d)", c->gamma);
+    if (c->nu1 < 0 || c->nu2 < 0) return fail("nu1/nu2 must be >= 0");
+    if (c->max_iters < 0) return fail("max_iters must be >= 0");
+    return 0;
+}
+
+template <typename T>
+Hier<T>& hier(jsv_plan* P);
+template <>
+Hier<double>& hier<double>(jsv_plan* P) { return P->h64; }
+template <>
+Hier<float>& hier<float>(jsv_plan* P) { return P->h32; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int jsv_abi_version(void) { return 1; }
+
+const char* jsv_last_error(void) { return g_err.c_str(); }
+
+int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double idel_x,
+                            double idel_y, double idel_z, int precision, jsv_plan** out)
+{
+    if (out == nullptr) return fail("out is NULL");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail("dim must be 2 or 3 (got %d)", dim);
+    if (batch < 1 || (dim == 3 && batch != 1))
+        return fail("batch must be >= 1 (2D) or 1 (3D), got %d", batch);
+    if (precision != JSV_FP64 && precision != JSV_MIXED) return fail("unknown precision %d", precision);
+    const int ms[3] = {mx, my, dim == 3 ? mz : 1};
+    const char names[3] = {'x', 'y', 'z'};
+    for (int a = 0; a < dim; ++a) {
+        if (ms[a] < 2 || (ms[a] & (ms[a] - 1)) != 0)
+            return fail("Resolution of %c-grid is not a power of 2. Value: %d", names[a], ms[a]);
+    }
+    int lv[3], L = 1 << 30;
+    for (int a = 0; a < dim; ++a) {
+        lv[a] = (int)std::lround(std::log2((double)ms[a])) + 1;
+        L = lv[a] < L ? lv[a] : L;
+    }
+    for (int a = 0; a < dim; ++a) {
+        const int coarsest = 1 << (lv[a] - L);
+        if (coarsest > 5)
+            return fail("Resolution of %c-grid not efficient for multigrid configuration. "
+                        "Resolution of coarsest level: %d", names[a], coarsest);
+    }
+    if (L < 2) return fail("grid too small for multigrid (levels=%d)", L);
+    jsv_plan* P = new jsv_plan();
+    P->dim = dim;
+    P->batch = batch;
+    P->prec = precision;
+    P->levels = L;
+    for (int a = 0; a < 3; ++a) P->m[a] = ms[a];
+    P->idel[0] = idel_x;
+    P->idel[1] = idel_y;
+    P->idel[2] = idel_z;
+    int rc = alloc_hier(P, P->h64, true);   // D_xy arrays are added by set_operator
+    if (rc == 0 && precision == JSV_MIXED) rc = alloc_hier(P, P->h32, true);
+    const Geo& g0 = P->h64.g[0];
+    if (rc == 0) rc = dalloc(P, (void**)&P->phi_old, sizeof(double) * g0.n);
+    if (rc == 0) {
+        P->npartials = (long)((g0.mx + jsv::BX - 1) / jsv::BX + 1) * ((g0.my + jsv::BY) / jsv::BY + 1) *
+                       (dim == 3 ? g0.mz + 1 : batch);
+        rc = dalloc(P, (void**)&P->partials, sizeof(double) * P->npartials);
+    }
+    if (rc == 0) rc = dalloc(P, (void**)&P->dscal, sizeof(double) * 4 * batch);
+    if (rc == 0) rc = dalloc(P, (void**)&P->d_act, sizeof(int) * batch);
+    if (rc == 0) {
+        hipError_t e = hipHostMalloc((void**)&P->hscal, sizeof(double) * 4 * batch, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&P->h_act, sizeof(int) * batch, hipHostMallocDefault);
+        if (e != hipSuccess) rc = fail("hipHostMalloc failed: %s", hipGetErrorString(e));
+    }
+    if (rc != 0) {
+        std::string keep = g_err;
+        jsv_plan_destroy(P);
+        g_err = keep;
+        return rc;
+    }
+    *out = P;
+    return 0;
+}
+
+int jsv_plan_create(int dim, int mx, int my, int mz, double idel_x, double idel_y, double idel_z,
+                    int precision, jsv_plan** out)
+{
+    return jsv_plan_create_batched(dim, mx, my, mz, 1, idel_x, idel_y, idel_z, precision, out);
+}
+
+int jsv_plan_destroy(jsv_plan* P)
+{
+    if (P == nullptr) return 0;
+    for (void* p : P->allocs) (void)hipFree(p);
+    if (P->hscal) (void)hipHostFree(P->hscal);
+    if (P->h_act) (void)hipHostFree(P->h_act);
+    delete P;
+    return 0;
+}
+
+int jsv_plan_levels(const jsv_plan* P, int* levels)
+{
+    if (P == nullptr || levels == nullptr) return fail("NULL argument");
+    *levels = P->levels;
+    return 0;
+}
+
+int jsv_plan_level_shape(const jsv_plan* P, int level, int shape[3])
+{
+    if (P == nullptr || shape == nullptr) return fail("NULL argument");
+    if (level < 0 || level >= P->levels) return fail("level %d out of range", level);
+    const Geo& g = P->h64.g[level];
+    shape[0] = g.nz;
+    shape[1] = g.ny;
+    shape[2] = g.nx;
+    return 0;
+}
+
+int jsv_plan_set_slab_bytes(jsv_plan* P, int64_t bytes)
+{
+    if (P == nullptr) return fail("plan is NULL");
+    if (bytes < 0) return fail("slab bytes must be >= 0");
+    P->slab_bytes = (long)bytes;
+    return 0;
+}
+
+int jsv_plan_device_bytes(const jsv_plan* P, int64_t* bytes)
+{
+    if (P == nullptr || bytes == nullptr) return fail("NULL argument");
+    *bytes = P->bytes;
+    return 0;
+}
+
+int jsv_plan_set_operator(jsv_plan* P, const double* lam, double lam_scalar, const double* dxx,
+                          const double* dyy, const double* dzz, const double d_scalar[3],
+                          const double* dxy, void* stream)
+{
+    if (P == nullptr) return fail("plan is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const int dim = P->dim;
+    const bool spatial = dxx != nullptr;
+    if (spatial && (dyy == nullptr || (dim == 3 && dzz == nullptr)))
+        return fail("D components must all be arrays or all be scalars");
+    if (!spatial && (dyy != nullptr || (dim == 3 && dzz != nullptr)))
+        return fail("D components must all be arrays or all be scalars");
+    if (!spatial && d_scalar == nullptr) return fail("scalar D values missing");
+    if (dxy != nullptr && (dim != 2 || !spatial)) return fail("D_xy needs 2D array diffusion");
+    P->spatial = spatial;
+    P->offdiag = dxy != nullptr;
+    Hier<double>& H = P->h64;
+    const int L = P->levels;
+    if (P->offdiag && H.drr[0] == nullptr) {
+        for (int l = 0; l < L; ++l) {
+            RETCHK(dalloc(P, (void**)&H.drr[l], sizeof(double) * H.g[l].n));
+            RETCHK(dalloc(P, (void**)&H.dlr[l], sizeof(double) * H.g[l].n));
+        }
+        if (P->prec == JSV_MIXED) {
+            for (int l = 0; l < L; ++l) {
+                RETCHK(dalloc(P, (void**)&P->h32.drr[l], sizeof(float) * H.g[l].n));
+                RETCHK(dalloc(P, (void**)&P->h32.dlr[l], sizeof(float) * H.g[l].n));
+            }
+        }
+    }
+    const Geo& g0 = H.g[0];
+    const double ds[3] = {d_scalar ? d_scalar[0] : 0.0, d_scalar ? d_scalar[1] : 0.0,
+                          d_scalar ? d_scalar[2] : 0.0};
+    if (dim == 2) {
+        for (int l = 0; l < L; ++l) {
+            const Geo& g = H.g[l];
+            const double idx = P->idel[0] * std::ldexp(1.0, -l);
+            const double idy = P->idel[1] * std::ldexp(1.0, -l);
+            const double sqx = idx * idx, sqy = idy * idy;
+            const double cxy = idx * idy * 0.25;
+            const int stride = 1 << l;
+            jsv::Src sDx{dxx, g0.sy, g0.sz, stride, l == 0};
+            jsv::Src sDy{dyy, g0.sy, g0.sz, stride, l == 0};
+            jsv::Src sLam{lam, g0.sy, g0.sz, stride, l == 0};
+            jsv::Src sDxy{dxy, g0.sy, g0.sz, stride, l == 0};
+            hipLaunchKernelGGL(jsv::k_setup2d, grid_exact(g.n), dim3(256), 0, s, H.cx[l], H.cy[l],
+                               H.cc[l], P->offdiag ? H.drr[l] : nullptr,
+                               P->offdiag ? H.dlr[l] : nullptr, g, sDx, sDy, sLam, sDxy, sqx, sqy,
+                               cxy, lam_scalar, ds[0], ds[1], spatial ? 1 : 0);
+            LAUNCHCHK();
+        }
+    } else {
+        // averaged face arrays per level (level 0: the raw input)
+        std::vector<double*> avg[3];
+        std::vector<void*> tmp;
+        int rc = 0;
+        if (spatial) {
+            for (int a = 0; a < 3 && rc == 0; ++a) {
+                avg[a].assign(L, nullptr);
+                avg[a][0] = const_cast<double*>(a == 0 ? dxx : a == 1 ? dyy : dzz);
+                for (int l = 1; l < L && rc == 0; ++l) {
+                    void* p = nullptr;
+                    hipError_t e = hipMallocAsync(&p, sizeof(double) * H.g[l].n, s);
+                    if (e != hipSuccess) rc = fail("hipMallocAsync failed: %s", hipGetErrorString(e));
+                    avg[a][l] = (double*)p;
+                    tmp.push_back(p);
+                    if (rc == 0)
+                        hipLaunchKernelGGL(jsv::k_avg3, grid_exact(H.g[l].n), dim3(256), 0, s,
+                                           avg[a][l], avg[a][l - 1], H.g[l], H.g[l - 1], a);
+                }
+            }
+        }
+        for (int l = 0; l < L && rc == 0; ++l) {
+            const Geo& g = H.g[l];
+            const double idx = P->idel[0] * std::ldexp(1.0, -l);
+            const double idy = P->idel[1] * std::ldexp(1.0, -l);
+            const double idz = P->idel[2] * std::ldexp(1.0, -l);
+            jsv::Src sLam{lam, g0.sy, g0.sz, 1 << l, l == 0};
+            hipLaunchKernelGGL(jsv::k_setup3d, grid_exact(g.n), dim3(256), 0, s, H.cx[l], H.cy[l],
+                               H.cz[l], H.cc[l], g, spatial ? avg[0][l] : nullptr,
+                               spatial ? avg[1][l] : nullptr, spatial ? avg[2][l] : nullptr, sLam,
+                               idx * idx, idy * idy, idz * idz, lam_scalar, ds[0], ds[1], ds[2],
+                               spatial ? 1 : 0);
+        }
+        for (void* p : tmp) (void)hipFreeAsync(p, s);
+        if (rc != 0) return rc;
+        LAUNCHCHK();
+    }
+    if (P->prec == JSV_MIXED) {
+        Hier<float>& F = P->h32;
+        for (int l = 0; l < L; ++l) {
+            const long n = H.g[l].n;
+            auto cast = [&](const double* a, float* b) {
+                if (a != nullptr && b != nullptr)
+                    hipLaunchKernelGGL((jsv::k_cast<double, float>), grid_flat(n), dim3(256), 0, s,
+                                       a, b, n);
+            };
+            cast(H.cx[l], F.cx[l]);
+            cast(H.cy[l], F.cy[l]);
+            if (dim == 3) cast(H.cz[l], F.cz[l]);
+            cast(H.cc[l], F.cc[l]);
+            if (P->offdiag) {
+                cast(H.drr[l], F.drr[l]);
+                cast(H.dlr[l], F.dlr[l]);
+            }
+        }
+        LAUNCHCHK();
+    }
+    P->op_set = true;
+    return 0;
+}
+
+int jsv_level_upload(jsv_plan* P, int prec, int level, int which, const void* src, void* stream)
+{
+    if (P == nullptr || src == nullptr) return fail("NULL argument");
+    if (level < 0 || level >= P->levels) return fail("level %d out of range", level);
+    if (prec == JSV_MIXED && P->prec != JSV_MIXED) return fail("plan has no FP32 hierarchy");
+    const long n = P->h64.g[level].n;
+    void* dst;
+    size_t es;
+    if (prec == JSV_MIXED) {
+        dst = which == JSV_BUF_PHI ? (void*)P->h32.phi[level] : (void*)P->h32.rhs[level];
+        es = sizeof(float);
+    } else {
+        dst = which == JSV_BUF_PHI ? (void*)P->h64.phi[level] : (void*)P->h64.rhs[level];
+        es = sizeof(double);
+    }
+    HIPCHK(hipMemcpyAsync(dst, src, es * n, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return 0;
+}
+
+int jsv_level_download(jsv_plan* P, int prec, int level, int which, void* dst, void* stream)
+{
+    if (P == nullptr || dst == nullptr) return fail("NULL argument");
+    if (level < 0 || level >= P->levels) return fail("level %d out of range", level);
+    if (prec == JSV_MIXED && P->prec != JSV_MIXED) return fail("plan has no FP32 hierarchy");
+    const long n = P->h64.g[level].n;
+    const void* src;
+    size_t es;
+    if (prec == JSV_MIXED) {
+        src = which == JSV_BUF_PHI ? (void*)P->h32.phi[level] : (void*)P->h32.rhs[level];
+        es = sizeof(float);
+    } else {
+        src = which == JSV_BUF_PHI ? (void*)P->h64.phi[level] : (void*)P->h64.rhs[level];
+        es = sizeof(double);
+    }
+    HIPCHK(hipMemcpyAsync(dst, src, es * n, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return 0;
+}
+
+int jsv_smooth(jsv_plan* P, int prec, int level, int sweeps, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (level < 0 || level >= P->levels) return fail("level %d out of range", level);
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < sweeps; ++i) {
+        if (prec == JSV_MIXED) RETCHK(smooth(P, P->h32, level, s));
+        else RETCHK(smooth(P, P->h64, level, s));
+    }
+    LAUNCHCHK();
+    return 0;
+}
+
+int jsv_restrict(jsv_plan* P, int prec, int level, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (level < 0 || level >= P->levels - 1) return fail("level %d out of range", level);
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == JSV_MIXED) RETCHK(restrict_(P, P->h32, level, s));
+    else RETCHK(restrict_(P, P->h64, level, s));
+    LAUNCHCHK();
+    return 0;
+}
+
+int jsv_prolong_add(jsv_plan* P, int prec, int level, int nu2, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (level < 0 || level >= P->levels - 1) return fail("level %d out of range", level);
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == JSV_MIXED) RETCHK(prolong(P, P->h32, level, nu2, s));
+    else RETCHK(prolong(P, P->h64, level, nu2, s));
+    LAUNCHCHK();
+    return 0;
+}
+
+int jsv_cycle(jsv_plan* P, int prec, const jsv_config* cfg, int skip_pre, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    RETCHK(check_cfg(cfg));
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == JSV_MIXED) {
+        if (P->prec != JSV_MIXED) return fail("plan has no FP32 hierarchy");
+        // rim of the FP32 level-0 phi: zero via the same kernel on a temp view is
+        // not available for float; the FP32 buffers are only written on unknowns
+        RETCHK(mg_cycle(P, P->h32, *cfg, 0, cfg->fcycle != 0, skip_pre != 0, s));
+    } else {
+        RETCHK(zero_rim_level0(P, P->h64.phi[0], s));
+        RETCHK(mg_cycle(P, P->h64, *cfg, 0, cfg->fcycle != 0, skip_pre != 0, s));
+    }
+    LAUNCHCHK();
+    return 0;
+}
+
+int jsv_residual(jsv_plan* P, const double* phi, const double* rhs, double* out_r,
+                 double* norm2_out, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (phi == nullptr || rhs == nullptr) return fail("NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    const long n = P->h64.g[0].n;
+    if (out_r != nullptr) RETCHK(zero(out_r, n, s));
+    // phi's boundary nodes and ghosts are the zero Dirichlet values: use a copy
+    // with the rim cleared, as the solver's own iterates always have it
+    HIPCHK(hipMemcpyAsync(P->phi_old, phi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    RETCHK(zero_rim_level0(P, P->phi_old, s));
+    RETCHK(residual64<double>(P, P->phi_old, rhs, out_r, 0, s));
+    LAUNCHCHK();
+    if (norm2_out != nullptr) {
+        RETCHK(read_scalars(P, 1, s));
+        for (int b = 0; b < nseg(P); ++b) norm2_out[b] = P->hscal[b];
+    }
+    return 0;
+}
+
+int jsv_max_abs_diff(const double* a, const double* b, int64_t n, double* out, void* stream)
+{
+    if (a == nullptr || b == nullptr || out == nullptr) return fail("NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* d = nullptr;
+    HIPCHK(hipMallocAsync((void**)&d, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(jsv::k_set_u64, dim3(1), dim3(64), 0, s, d, 0ull);
+    long nb = (long)((n + 255) / 256);
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(jsv::k_maxabsdiff<double>, dim3((unsigned)nb), dim3(256), 0, s, a, b,
+                       (long)n, d);
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFreeAsync(d, s);
+    if (e != hipSuccess) return fail("max_abs_diff failed: %s", hipGetErrorString(e));
+    double v;
+    std::memcpy(&v, &h, sizeof(v));
+    *out = v;
+    return 0;
+}
+
+static int do_bcs(double* arr, int dim, int nz, int ny, int nx, int rim, int direction,
+                  const double* vxb, const double* vxe, const double* vyb, const double* vye,
+                  void* stream)
+{
+    if (arr == nullptr) return fail("NULL array");
+    if (rim < 0) return fail("rim must be >= 0");
+    Geo g{};
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.sy = nx;
+    g.sz = (long)nx * ny;
+    g.n = g.sz * nz;
+    hipStream_t s = (hipStream_t)stream;
+    for (int ph = 0; ph < dim; ++ph)
+        hipLaunchKernelGGL(jsv::k_bcs, grid_exact(g.n), dim3(256), 0, s, arr, g, dim, rim,
+                           direction, vxb, vxe, vyb, vye, ph);
+    LAUNCHCHK();
+    return 0;
+}
+
+int jsv_do_bcs_2d(double* arr, int ny, int nx, int rim, int direction, const double* vals_xb,
+                  const double* vals_xe, const double* vals_yb, const double* vals_ye, void* stream)
+{
+    return do_bcs(arr, 2, 1, ny, nx, rim, direction, vals_xb, vals_xe, vals_yb, vals_ye, stream);
+}
+
+int jsv_do_bcs_3d(double* arr, int nz, int ny, int nx, int rim, int direction,
+                  const double* vals_xb, const double* vals_xe, const double* vals_yb,
+                  const double* vals_ye, void* stream)
+{
+    return do_bcs(arr, 3, nz, ny, nx, rim, direction, vals_xb, vals_xe, vals_yb, vals_ye, stream);
+}
+
+// ---------------------------------------------------------------------------
+// solve loop (Solver_2D.solve, solver_2d.py:423-479)
+// ---------------------------------------------------------------------------
+static void record(const jsv_config* c, int64_t k, double v)
+{
+    if (c->history != nullptr && k < c->history_len) c->history[k] = v;
+}
+
+// Per-problem bookkeeping of a (batched) dynamic solve.  A problem stops
+// after the first cycle whose stop quantity is not above the threshold (NaN
+// stops it too, as `dist > eps` is false for NaN in the reference); kernels of
+// later cycles skip it through the device flags.
+struct Track {
+    std::vector<int64_t> count, outer;
+    std::vector<double> last, rel;
+    std::vector<int> nonfinite;
+    int active = 0;
+    void init(int B)
+    {
+        count.assign(B, 0);
+        outer.assign(B, 0);
+        last.assign(B, INFINITY);
+        rel.assign(B, -1.0);
+        nonfinite.assign(B, 0);
+        active = B;
+    }
+};
+
+static int push_act(jsv_plan* P, hipStream_t s)
+{
+    HIPCHK(hipMemcpyAsync(P->d_act, P->h_act, sizeof(int) * P->batch, hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+static void fill_result(const Track& T, jsv_result* res, int64_t* counts, double* stopvals, int B)
+{
+    int64_t mx = 0, mo = 0;
+    int nf = 0;
+    for (int b = 0; b < B; ++b) {
+        mx = T.count[b] > mx ? T.count[b] : mx;
+        mo = T.outer[b] > mo ? T.outer[b] : mo;
+        nf |= T.nonfinite[b];
+        if (counts) counts[b] = T.count[b];
+        if (stopvals) stopvals[b] = T.last[b];
+    }
+    res->count = B == 1 ? T.count[0] : mx;
+    res->outer_steps = B == 1 ? T.outer[0] : mo;
+    res->nonfinite = nf;
+    res->rel_residual = T.rel[0];
+}
+
+static int solve_fp64(jsv_plan* P, const double* phi_in, const double* rhs, double* phi_out,
+                      const jsv_config* c, jsv_result* res, int64_t* counts, double* stopvals,
+                      hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    const long n = H.g[0].n;
+    const int B = nseg(P);
+    HIPCHK(hipMemcpyAsync(H.rhs[0], rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(H.phi[0], phi_in, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    RETCHK(zero_rim_level0(P, H.phi[0], s));
+    const bool relres = !c->fixed && c->stop == JSV_STOP_REL_RESIDUAL;
+    std::vector<double> rhs_norm(B, 0.0);
+    if (relres) {
+        RETCHK(norm2_unknowns<double>(P, H.rhs[0], 1, s));
+        RETCHK(read_scalars(P, 2, s));
+        for (int b = 0; b < B; ++b) rhs_norm[b] = std::sqrt(P->hscal[B + b]);
+    }
+    Track T;
+    T.init(B);
+    for (int b = 0; b < B; ++b) P->h_act[b] = 1;
+    RETCHK(push_act(P, s));
+    P->use_act = !c->fixed && B > 1;
+    // reference: initial skip_pre smoothing on the incoming phi
+    const double* first_old = phi_in;
+    if (c->skip_pre) {
+        for (int i = 0; i < c->nu1; ++i) RETCHK(smooth(P, H, 0, s));
+        HIPCHK(hipMemcpyAsync(P->phi_old, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        first_old = P->phi_old;
+    }
+    int rc = 0;
+    if (c->fixed) {
+        for (int64_t k = 0; k < c->max_iters && rc == 0; ++k) rc = mg_cycle(P, H, *c, 0, c->fcycle != 0,
+                                                                           c->skip_pre != 0, s);
+        for (int b = 0; b < B; ++b) T.count[b] = T.outer[b] = c->max_iters;
+    } else {
+        const double thr = relres ? c->rtol : c->epsilon;
+        for (int b = 0; b < B; ++b)
+            if (c->max_iters <= 0) { P->h_act[b] = 0; --T.active; }
+        int64_t it = 0;
+        while (T.active > 0 && rc == 0) {
+            const double* old = (it == 0) ? first_old : P->phi_old;
+            if (!relres && it > 0)
+                HIPCHK(hipMemcpyAsync(P->phi_old, H.phi[0], sizeof(double) * n,
+                                      hipMemcpyDeviceToDevice, s));
+            rc = mg_cycle(P, H, *c, 0, c->fcycle != 0, c->skip_pre != 0, s);
+            if (rc) break;
+            LAUNCHCHK();
+            if (relres) RETCHK(residual64<double>(P, H.phi[0], H.rhs[0], (double*)nullptr, 0, s));
+            else RETCHK(maxdiff(P, old, H.phi[0], 0, s));
+            RETCHK(read_scalars(P, 1, s));
+            bool changed = false;
+            for (int b = 0; b < B; ++b) {
+                if (!P->h_act[b]) continue;
+                const double v = relres ? std::sqrt(P->hscal[b]) / rhs_norm[b] : P->hscal[b];
+                T.last[b] = v;
+                if (relres) T.rel[b] = v;
+                if (!std::isfinite(v)) T.nonfinite[b] = 1;
+                if (b == 0) record(c, T.count[0], v);
+                ++T.count[b];
+                T.outer[b] = T.count[b];
+                if (!(v > thr) || T.count[b] >= c->max_iters) {
+                    P->h_act[b] = 0;
+                    --T.active;
+                    changed = true;
+                }
+            }
+            if (changed && T.active > 0) RETCHK(push_act(P, s));
+            ++it;
+        }
+    }
+    P->use_act = false;
+    if (rc) return rc;
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(phi_out, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    fill_result(T, res, counts, stopvals, B);
+    res->distance = relres ? -1.0 : T.last[0];
+    return 0;
+}
+
+// FP32 cycles inside FP64 iterative refinement
+static int solve_mixed(jsv_plan* P, const double* phi_in, const double* rhs, double* phi_out,
+                       const jsv_config* c, jsv_result* res, int64_t* counts, double* stopvals,
+                       hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    Hier<float>& F = P->h32;
+    const Geo& g = H.g[0];
+    const long n = g.n;
+    const int B = nseg(P);
+    const int k_inner = c->inner_cycles > 0 ? c->inner_cycles : 4;
+    HIPCHK(hipMemcpyAsync(H.rhs[0], rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(H.phi[0], phi_in, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    RETCHK(zero_rim_level0(P, H.phi[0], s));
+    RETCHK(norm2_unknowns<double>(P, H.rhs[0], 1, s));
+    RETCHK(read_scalars(P, 2, s));
+    std::vector<double> rhs_norm(B);
+    for (int b = 0; b < B; ++b) rhs_norm[b] = std::sqrt(P->hscal[B + b]);
+    const bool relres = c->fixed || c->stop == JSV_STOP_REL_RESIDUAL;
+    const double thr = c->fixed ? -1.0 : (relres ? c->rtol : c->epsilon);
+    dim3 gr = grid_points(g, P->dim, g.mx - 1);
+    Track T;
+    T.init(B);
+    for (int b = 0; b < B; ++b) P->h_act[b] = 1;
+    RETCHK(push_act(P, s));
+    P->use_act = B > 1;
+    int rc = 0;
+    bool first = true;
+    while (rc == 0) {
+        // FP64 residual, stored as the FP32 right-hand side of the correction equation
+        RETCHK(residual64<float>(P, H.phi[0], H.rhs[0], F.rhs[0], 0, s));
+        RETCHK(read_scalars(P, 1, s));
+        bool changed = false;
+        for (int b = 0; b < B; ++b) {
+            if (!P->h_act[b]) continue;
+            const double rel = std::sqrt(P->hscal[b]) / rhs_norm[b];
+            T.rel[b] = rel;
+            if (!std::isfinite(rel)) T.nonfinite[b] = 1;
+            if (relres) {
+                T.last[b] = rel;
+                if (b == 0) record(c, T.outer[0], rel);
+            }
+            const bool stop = relres ? !(rel > thr) : (!first && !(T.last[b] > thr));
+            if (stop || T.count[b] >= c->max_iters) {
+                P->h_act[b] = 0;
+                --T.active;
+                changed = true;
+            }
+        }
+        first = false;
+        if (T.active == 0) break;
+        if (changed) RETCHK(push_act(P, s));
+        // inner FP32 cycles on the correction, from a zero start
+        RETCHK(zero(F.phi[0], n, s));
+        int64_t todo = k_inner;
+        for (int b = 0; b < B; ++b)
+            if (P->h_act[b] && T.count[b] + todo > c->max_iters) todo = c->max_iters - T.count[b];
+        for (int64_t k = 0; k < todo && rc == 0; ++k) rc = mg_cycle(P, F, *c, 0, c->fcycle != 0, false, s);
+        if (rc) break;
+        if (!relres)
+            HIPCHK(hipMemcpyAsync(P->phi_old, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(jsv::k_correct<float>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[0], F.phi[0],
+                           g, P->dim, P->act_ptr());
+        LAUNCHCHK();
+        for (int b = 0; b < B; ++b)
+            if (P->h_act[b]) {
+                T.count[b] += todo;
+                ++T.outer[b];
+            }
+        if (!relres) {
+            RETCHK(maxdiff(P, P->phi_old, H.phi[0], 0, s));
+            RETCHK(read_scalars(P, 1, s));
+            for (int b = 0; b < B; ++b) {
+                if (!P->h_act[b]) continue;
+                T.last[b] = P->hscal[b];
+                if (!std::isfinite(T.last[b])) T.nonfinite[b] = 1;
+                if (b == 0) record(c, T.outer[0] - 1, T.last[0]);
+            }
+        }
+    }
+    P->use_act = false;
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(phi_out, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    fill_result(T, res, counts, stopvals, B);
+    res->distance = relres ? -1.0 : T.last[0];
+    return 0;
+}
+
+int jsv_solve_batched(jsv_plan* P, const double* phi_in, const double* rhs, double* phi_out,
+                      const jsv_config* cfg, jsv_result* res, int64_t* counts, double* stopvals,
+                      void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (phi_in == nullptr || rhs == nullptr || phi_out == nullptr || res == nullptr)
+        return fail("NULL argument");
+    RETCHK(check_cfg(cfg));
+    hipStream_t s = (hipStream_t)stream;
+    if (P->prec == JSV_MIXED) return solve_mixed(P, phi_in, rhs, phi_out, cfg, res, counts, stopvals, s);
+    return solve_fp64(P, phi_in, rhs, phi_out, cfg, res, counts, stopvals, s);
+}
+
+int jsv_solve(jsv_plan* P, const double* phi_in, const double* rhs, double* phi_out,
+              const jsv_config* cfg, jsv_result* res, void* stream)
+{
+    return jsv_solve_batched(P, phi_in, rhs, phi_out, cfg, res, nullptr, nullptr, stream);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Reverse pass of fixed-mode solves (FP64)
+// ===========================================================================
+#include "jsv_adjoint.hpp"
+
+namespace {
+
+struct Adj {
+    bool ready = false;
+    int64_t bytes = 0;
+    std::vector<double*> g, t, rb, rbar, bx, by, bz, bc, chain;
+    std::vector<std::vector<double*>> pool;   // per level snapshot slots
+    std::vector<size_t> used;
+    std::vector<std::pair<int, double*>> tape;
+};
+
+// snapshots per level for one cycle (mirrors mg_cycle)
+void count_tape(const jsv_plan* P, const jsv_config& c, int l, bool fcyc, bool skip_pre,
+                std::vector<long>& cnt)
+{
+    if (l == P->levels - 2) {
+        cnt[l] += 2L * (c.nu1 + c.nu2);
+        return;
+    }
+    if (!skip_pre) cnt[l] += 2L * c.nu1;
+    cnt[l] += 1;   // before the restriction
+    if (fcyc)
+        for (int i = 0; i < c.gamma; ++i) count_tape(P, c, l + 1, true, false, cnt);
+    for (int i = 0; i < c.gamma; ++i) count_tape(P, c, l + 1, false, false, cnt);
+    cnt[l] += 2L * c.nu2;
+}
+
+int adj_alloc(jsv_plan* P, Adj& A, const jsv_config& c)
+{
+    const int L = P->levels;
+    const Hier<double>& H = P->h64;
+    if (!A.ready) {
+        auto one = [&](std::vector<double*>& v, bool need) -> int {
+            v.assign(L, nullptr);
+            if (!need) return 0;
+            for (int l = 0; l < L; ++l) {
+                RETCHK(dalloc(P, (void**)&v[l], sizeof(double) * H.g[l].n));
+                A.bytes += (int64_t)sizeof(double) * H.g[l].n;
+            }
+            return 0;
+        };
+        RETCHK(one(A.g, true));
+        RETCHK(one(A.t, true));
+        RETCHK(one(A.rb, true));
+        RETCHK(one(A.rbar, true));
+        RETCHK(one(A.bx, true));
+        RETCHK(one(A.by, true));
+        RETCHK(one(A.bz, P->dim == 3));
+        RETCHK(one(A.bc, true));
+        RETCHK(one(A.chain, true));
+        A.pool.assign(L, {});
+        A.ready = true;
+    }
+    std::vector<long> cnt(L, 0);
+    count_tape(P, c, 0, c.fcycle != 0, c.skip_pre != 0, cnt);
+    // the optional pre-loop smoothing (skip_pre) also records on level 0
+    if (c.skip_pre) cnt[0] = cnt[0] > 2L * c.nu1 ? cnt[0] : 2L * c.nu1;
+    for (int l = 0; l < L; ++l) {
+        while ((long)A.pool[l].size() < cnt[l]) {
+            double* p = nullptr;
+            RETCHK(dalloc(P, (void**)&p, sizeof(double) * H.g[l].n));
+            A.bytes += (int64_t)sizeof(double) * H.g[l].n;
+            A.pool[l].push_back(p);
+        }
+    }
+    A.used.assign(L, 0);
+    A.tape.clear();
+    return 0;
+}
+
+int snap(jsv_plan* P, Adj& A, int l, hipStream_t s)
+{
+    if (A.used[l] >= A.pool[l].size()) return fail("adjoint tape overflow at level %d", l);
+    double* dst = A.pool[l][A.used[l]++];
+    HIPCHK(hipMemcpyAsync(dst, P->h64.phi[l], sizeof(double) * P->h64.g[l].n,
+                          hipMemcpyDeviceToDevice, s));
+    A.tape.push_back({l, dst});
+    return 0;
+}
+
+int first_parity(const jsv_plan* P) { return P->dim == 3 ? 1 : 0; }
+
+int rec_half(jsv_plan* P, Adj& A, int l, int par, hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    const Geo& g = H.g[l];
+    dim3 gr = grid_points(g, P->dim, g.mx / 2);
+    if (P->dim == 2)
+        hipLaunchKernelGGL(jsv::k_rb2<double>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
+                           H.cx[l], H.cy[l], H.cc[l], g, par, (const int*)nullptr);
+    else
+        hipLaunchKernelGGL(jsv::k_rb3<double>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
+                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, par, 2);
+    return snap(P, A, l, s);
+}
+
+int rec_smooth(jsv_plan* P, Adj& A, int l, hipStream_t s)
+{
+    const int f = first_parity(P);
+    RETCHK(rec_half(P, A, l, f, s));
+    return rec_half(P, A, l, 1 - f, s);
+}
+
+int rec_cycle(jsv_plan* P, Adj& A, const jsv_config& c, int l, bool fcyc, bool skip_pre, hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    if (l == P->levels - 2) {
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) RETCHK(rec_smooth(P, A, l, s));
+        return 0;
+    }
+    if (!skip_pre)
+        for (int i = 0; i < c.nu1; ++i) RETCHK(rec_smooth(P, A, l, s));
+    RETCHK(snap(P, A, l, s));
+    RETCHK(restrict_(P, H, l, s));
+    RETCHK(zero(H.phi[l + 1], H.g[l + 1].n, s));
+    if (fcyc)
+        for (int i = 0; i < c.gamma; ++i) RETCHK(rec_cycle(P, A, c, l + 1, true, false, s));
+    for (int i = 0; i < c.gamma; ++i) RETCHK(rec_cycle(P, A, c, l + 1, false, false, s));
+    RETCHK(prolong(P, H, l, c.nu2, s));
+    for (int i = 0; i < c.nu2; ++i) RETCHK(rec_smooth(P, A, l, s));
+    return 0;
+}
+
+double* pop(Adj& A, int l)
+{
+    auto e = A.tape.back();
+    A.tape.pop_back();
+    return e.first == l ? e.second : nullptr;
+}
+
+int adj_half(jsv_plan* P, Adj& A, int l, int par, const double* S, hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    const Geo& g = H.g[l];
+    dim3 gr = grid_points(g, P->dim, g.mx / 2);
+    hipLaunchKernelGGL(jsv::k_adj_half_a, gr, dim3(jsv::BX, jsv::BY), 0, s, A.g[l], A.t[l], S,
+                       H.cc[l], A.rbar[l], A.bx[l], A.by[l], P->dim == 3 ? A.bz[l] : nullptr,
+                       A.bc[l], g, par, P->dim);
+    hipLaunchKernelGGL(jsv::k_adj_half_b, gr, dim3(jsv::BX, jsv::BY), 0, s, A.g[l], A.t[l], H.cx[l],
+                       H.cy[l], P->dim == 3 ? H.cz[l] : nullptr, g, 1 - par, P->dim);
+    return 0;
+}
+
+int adj_smooth(jsv_plan* P, Adj& A, int l, hipStream_t s)
+{
+    const int f = first_parity(P);
+    double* sb = pop(A, l);
+    double* sa = pop(A, l);
+    if (sa == nullptr || sb == nullptr) return fail("adjoint tape mismatch at level %d", l);
+    RETCHK(adj_half(P, A, l, 1 - f, sb, s));
+    return adj_half(P, A, l, f, sa, s);
+}
+
+int adj_cycle(jsv_plan* P, Adj& A, const jsv_config& c, int l, bool fcyc, bool skip_pre, hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    if (l == P->levels - 2) {
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) RETCHK(adj_smooth(P, A, l, s));
+        return 0;
+    }
+    for (int i = 0; i < c.nu2; ++i) RETCHK(adj_smooth(P, A, l, s));
+    const Geo& gf = H.g[l];
+    const Geo& gc = H.g[l + 1];
+    {
+        const int full = (c.nu2 <= 0 || (P->dim == 2 && P->offdiag)) ? 1 : 0;
+        dim3 gr = grid_points(gc, P->dim, gc.mx - 1);
+        hipLaunchKernelGGL(jsv::k_adj_prolong, gr, dim3(jsv::BX, jsv::BY), 0, s, A.g[l + 1], A.g[l],
+                           gf, gc, P->dim, full);
+    }
+    RETCHK(zero(A.rbar[l + 1], gc.n, s));
+    for (int i = 0; i < c.gamma; ++i) RETCHK(adj_cycle(P, A, c, l + 1, false, false, s));
+    if (fcyc)
+        for (int i = 0; i < c.gamma; ++i) RETCHK(adj_cycle(P, A, c, l + 1, true, false, s));
+    // transposed restriction with the state saved before it
+    double* S = pop(A, l);
+    if (S == nullptr) return fail("adjoint tape mismatch (restrict) at level %d", l);
+    {
+        const int f = first_parity(P);
+        dim3 gr = grid_points(gf, P->dim, gf.mx - 1);
+        hipLaunchKernelGGL(jsv::k_adj_restrict_rbar, gr, dim3(jsv::BX, jsv::BY), 0, s, A.rb[l],
+                           A.rbar[l + 1], gf, gc, P->dim, f);
+        hipLaunchKernelGGL(jsv::k_adj_restrict_apply, gr, dim3(jsv::BX, jsv::BY), 0, s, A.g[l], A.rb[l],
+                           S, H.cx[l], H.cy[l], P->dim == 3 ? H.cz[l] : nullptr, H.cc[l], A.rbar[l],
+                           A.bx[l], A.by[l], P->dim == 3 ? A.bz[l] : nullptr, A.bc[l], gf, P->dim, f);
+    }
+    if (!skip_pre)
+        for (int i = 0; i < c.nu1; ++i) RETCHK(adj_smooth(P, A, l, s));
+    return 0;
+}
+
+int zero_rim_any(const Geo& g, int dim, double* a, hipStream_t s)
+{
+    for (int ph = 0; ph < dim; ++ph)
+        hipLaunchKernelGGL(jsv::k_bcs, grid_exact(g.n), dim3(256), 0, s, a, g, dim, 1, -1, nullptr,
+                           nullptr, nullptr, nullptr, ph);
+    return 0;
+}
+
+Adj& adj_of(jsv_plan* P);
+
+}  // namespace
+
+// per-plan adjoint state, created on first use
+#include <map>
+namespace {
+std::map<const jsv_plan*, Adj> g_adj;
+Adj& adj_of(jsv_plan* P) { return g_adj[P]; }
+}  // namespace
+
+extern "C" {
+
+int jsv_forward_checkpoints(jsv_plan* P, const double* phi0, const double* rhs, const jsv_config* c,
+                            double* ckpt, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (phi0 == nullptr || rhs == nullptr || ckpt == nullptr) return fail("NULL argument");
+    if (P->batch != 1) return fail("checkpointed solves need an unbatched plan");
+    RETCHK(check_cfg(c));
+    hipStream_t s = (hipStream_t)stream;
+    Hier<double>& H = P->h64;
+    const long n = H.g[0].n;
+    HIPCHK(hipMemcpyAsync(H.rhs[0], rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(H.phi[0], phi0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    RETCHK(zero_rim_level0(P, H.phi[0], s));
+    if (c->skip_pre)
+        for (int i = 0; i < c->nu1; ++i) RETCHK(smooth(P, H, 0, s));
+    HIPCHK(hipMemcpyAsync(ckpt, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    for (int64_t k = 0; k < c->max_iters; ++k) {
+        RETCHK(zero_rim_level0(P, H.phi[0], s));
+        RETCHK(mg_cycle(P, H, *c, 0, c->fcycle != 0, c->skip_pre != 0, s));
+        HIPCHK(hipMemcpyAsync(ckpt + (k + 1) * n, H.phi[0], sizeof(double) * n,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    LAUNCHCHK();
+    return 0;
+}
+
+int jsv_adjoint(jsv_plan* P, const double* phi0, const double* rhs, const double* ckpt,
+                const jsv_config* c, const double* g_out, double* g_phi0, double* g_rhs,
+                double* g_lam, double* g_dxx, double* g_dyy, double* g_dzz, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (phi0 == nullptr || rhs == nullptr || ckpt == nullptr || g_out == nullptr)
+        return fail("NULL argument");
+    if (P->offdiag) return fail("the reverse pass does not support D_xy");
+    if (P->batch != 1) return fail("the reverse pass needs an unbatched plan");
+    RETCHK(check_cfg(c));
+    hipStream_t s = (hipStream_t)stream;
+    Hier<double>& H = P->h64;
+    Adj& A = adj_of(P);
+    RETCHK(adj_alloc(P, A, *c));
+    const int L = P->levels;
+    const long n = H.g[0].n;
+    for (int l = 0; l < L; ++l) {
+        const long nl = H.g[l].n;
+        RETCHK(zero(A.rbar[l], nl, s));
+        RETCHK(zero(A.bx[l], nl, s));
+        RETCHK(zero(A.by[l], nl, s));
+        if (P->dim == 3) RETCHK(zero(A.bz[l], nl, s));
+        RETCHK(zero(A.bc[l], nl, s));
+        RETCHK(zero(A.t[l], nl, s));
+        RETCHK(zero(A.rb[l], nl, s));
+        RETCHK(zero(A.g[l], nl, s));
+    }
+    HIPCHK(hipMemcpyAsync(H.rhs[0], rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(A.g[0], g_out, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    RETCHK(zero_rim_any(H.g[0], P->dim, A.g[0], s));
+    for (int64_t k = c->max_iters; k >= 1; --k) {
+        // recompute cycle k from its input with a tape, then reverse it
+        A.used.assign(L, 0);
+        A.tape.clear();
+        HIPCHK(hipMemcpyAsync(H.phi[0], ckpt + (k - 1) * n, sizeof(double) * n,
+                              hipMemcpyDeviceToDevice, s));
+        RETCHK(rec_cycle(P, A, *c, 0, c->fcycle != 0, c->skip_pre != 0, s));
+        RETCHK(adj_cycle(P, A, *c, 0, c->fcycle != 0, c->skip_pre != 0, s));
+        if (!A.tape.empty()) return fail("adjoint tape not consumed (%zu left)", A.tape.size());
+        RETCHK(zero_rim_any(H.g[0], P->dim, A.g[0], s));   // cycle input passes do_BCs
+        LAUNCHCHK();
+    }
+    if (c->skip_pre && c->nu1 > 0) {
+        A.used.assign(L, 0);
+        A.tape.clear();
+        HIPCHK(hipMemcpyAsync(H.phi[0], phi0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        RETCHK(zero_rim_level0(P, H.phi[0], s));
+        for (int i = 0; i < c->nu1; ++i) RETCHK(rec_smooth(P, A, 0, s));
+        for (int i = 0; i < c->nu1; ++i) RETCHK(adj_smooth(P, A, 0, s));
+        RETCHK(zero_rim_any(H.g[0], P->dim, A.g[0], s));
+    }
+    if (g_phi0) HIPCHK(hipMemcpyAsync(g_phi0, A.g[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (g_rhs) HIPCHK(hipMemcpyAsync(g_rhs, A.rbar[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    // ---- coefficient chain ------------------------------------------------
+    const Geo& g0 = H.g[0];
+    if (g_lam) {
+        RETCHK(zero(g_lam, n, s));
+        for (int l = 0; l < L; ++l)
+            hipLaunchKernelGGL(jsv::k_adj_inject_to_fine, grid_exact(H.g[l].n), dim3(256), 0, s, g_lam,
+                               A.bc[l], H.g[l], g0, 1 << l, P->dim);
+    }
+    double* gD[3] = {g_dxx, g_dyy, g_dzz};
+    std::vector<double*> bface[3] = {A.bx, A.by, A.bz};
+    for (int a = 0; a < P->dim; ++a) {
+        if (gD[a] == nullptr) continue;
+        RETCHK(zero(gD[a], n, s));
+        for (int l = 0; l < L; ++l) {
+            const double id = P->idel[a] * std::ldexp(1.0, -l);
+            const double sq = id * id;
+            const Geo& g = H.g[l];
+            const long step = a == 0 ? 1 : a == 1 ? g.sy : g.sz;
+            hipLaunchKernelGGL(jsv::k_adj_face_total, grid_exact(g.n), dim3(256), 0, s, A.t[l],
+                               bface[a][l], A.bc[l], g, step, sq, 0);
+        }
+        if (P->dim == 2) {
+            for (int l = 0; l < L; ++l) {
+                const Geo& g = H.g[l];
+                hipLaunchKernelGGL(jsv::k_adj_average, grid_exact(g.n), dim3(256), 0, s, A.chain[l],
+                                   A.t[l], g, 2, a);
+                hipLaunchKernelGGL(jsv::k_adj_inject_to_fine, grid_exact(g.n), dim3(256), 0, s, gD[a],
+                                   A.chain[l], g, g0, 1 << l, 2);
+            }
+        } else {
+            // A.t[l] holds the bar of level l's face-averaged D; walk from the
+            // coarsest level down: bar(avg_l) -> raw_l -> avg_{l-1}
+            for (int l = L - 1; l >= 1; --l) {
+                const Geo& g = H.g[l];
+                hipLaunchKernelGGL(jsv::k_adj_average, grid_exact(g.n), dim3(256), 0, s, A.chain[l],
+                                   A.t[l], g, 3, a);
+                hipLaunchKernelGGL(jsv::k_adj_inject_one, grid_exact(g.n), dim3(256), 0, s,
+                                   A.t[l - 1], A.chain[l], g, H.g[l - 1]);
+            }
+            HIPCHK(hipMemcpyAsync(gD[a], A.t[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    LAUNCHCHK();
+    return 0;
+}
+
+int jsv_adjoint_device_bytes(const jsv_plan* P, int64_t* bytes)
+{
+    if (P == nullptr || bytes == nullptr) return fail("NULL argument");
+    auto it = g_adj.find(P);
+    *bytes = it == g_adj.end() ? 0 : it->second.bytes;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,660 @@
+// Device kernels of the multigrid path for gfx950 (CDNA4, wave64).
+//
+// Layout: every level stores phi, rhs and its coefficients as ghost-padded
+// row-major arrays of the same extents (nz, ny, nx) = (mz+3, my+3, mx+3); in 2D
+// nz = 1.  Coefficients use the phi index of the point they belong to:
+//   cx[p]  face coefficient between p and p+x (the reference's AMat_rc /
+//          AMat_rcc at p; its AMat_lc / AMat_lcc at p is cx[p-1]),
+//   cy[p], cz[p] likewise along y and z,
+//   cc[p]  the diagonal AMat_cc / AMat_ccc,
+//   drr[p] 2D off-diagonal coupling p <-> p+(1,1) (AMat_rr; AMat_ll at p is
+//          drr[p-(1,1)]), dlr[p] coupling p <-> p+(1,-1) in (y, x) (AMat_lr;
+//          AMat_rl at p is dlr[p+(-1,+1)]).
+// Storing the shared faces once (instead of the reference's separate left and
+// right arrays) is bit-identical: both are the same product D_face * idel^2.
+//
+// Arithmetic follows the reference's expressions term by term and is compiled
+// without FMA contraction, so the FP64 kernels reproduce an IEEE evaluation of
+// the reference formulas exactly (the CPU oracle does the same in NumPy).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace jsv {
+
+struct Geo {
+    int nx, ny, nz;   // padded extents; in 2D nz = number of batched problems
+    int mx, my, mz;   // resolutions (mz = 0 in 2D)
+    long sy, sz;      // strides in elements (2D: sz = one problem's array)
+    long n;           // nx * ny * nz
+};
+
+constexpr int BX = 64;   // one wave across x
+constexpr int BY = 4;    // 4 waves per workgroup
+
+// ---------------------------------------------------------------------------
+// Red-black Gauss-Seidel half sweeps (solver_2d.py:670-692, solver_3d.py:679-706).
+// The reference's sub-colours split into two independent halves: in 2D the
+// sub-lattices (0,0),(1,1) are the points with x+y even and (0,1),(1,0) those
+// with x+y odd; in 3D the first four sub-colours are the points with x+y+z odd,
+// the last four those with x+y+z even.  Points of one half never neighbour
+// each other in the 5/7-point stencil, so one launch per half reproduces the
+// reference's sequential sub-sweeps exactly.  Boundary nodes (index 1, m+1) and
+// ghosts are never written and stay zero, which is what the reference's
+// do_BCs between sub-sweeps guarantees.
+// ---------------------------------------------------------------------------
+// 2D kernels: blockIdx.z is the problem of a batch; act (nullable) holds a
+// per-problem "still iterating" flag.
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_rb2(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
+      const T* __restrict__ cy, const T* __restrict__ cc, Geo g, int par, const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    if (y > g.my) return;
+    const int x = 2 + ((par + y) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (x > g.mx) return;
+    const long p = (long)b * g.sz + (long)y * g.sy + x;
+    T acc = cy[p] * phi[p + g.sy] + cy[p - g.sy] * phi[p - g.sy];
+    acc = acc + cx[p] * phi[p + 1];
+    acc = acc + cx[p - 1] * phi[p - 1];
+    acc = acc - rhs[p];
+    phi[p] = acc / cc[p];
+}
+
+// 2D with D_xy: the 9-point stencil couples the sub-lattices diagonally, so
+// each of the reference's four sub-colours (lx, ly) is its own launch.
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_rb2_diag(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
+           const T* __restrict__ cy, const T* __restrict__ cc, const T* __restrict__ drr,
+           const T* __restrict__ dlr, Geo g, int lx, int ly, const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int y = (ly ? 2 : 3) + 2 * (blockIdx.y * BY + threadIdx.y);
+    if (y > g.my) return;
+    const int x = (lx ? 2 : 3) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (x > g.mx) return;
+    const long p = (long)b * g.sz + (long)y * g.sy + x;
+    const long sy = g.sy;
+    T acc = cy[p] * phi[p + sy] + cy[p - sy] * phi[p - sy];
+    acc = acc + cx[p] * phi[p + 1];
+    acc = acc + cx[p - 1] * phi[p - 1];
+    acc = acc - rhs[p];
+    T off = drr[p - sy - 1] * phi[p - sy - 1] + dlr[p - sy + 1] * phi[p - sy + 1];
+    off = off + dlr[p] * phi[p + sy - 1];
+    off = off + drr[p] * phi[p + sy + 1];
+    acc = acc + off;
+    phi[p] = acc / cc[p];
+}
+
+// z0: first plane of the launch (grid.z planes from z0); the slab schedule
+// launches one z-slab at a time.
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
+      const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc, Geo g, int par,
+      int z0)
+{
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int z = z0 + blockIdx.z;
+    if (y > g.my) return;
+    const int x = 2 + ((par + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (x > g.mx) return;
+    const long p = (long)z * g.sz + (long)y * g.sy + x;
+    T acc = cz[p] * phi[p + g.sz] + cz[p - g.sz] * phi[p - g.sz];
+    acc = acc + cy[p] * phi[p + g.sy];
+    acc = acc + cy[p - g.sy] * phi[p - g.sy];
+    acc = acc + cx[p] * phi[p + 1];
+    acc = acc + cx[p - 1] * phi[p - 1];
+    acc = acc - rhs[p];
+    phi[p] = acc / cc[p];
+}
+
+// ---------------------------------------------------------------------------
+// Residual rhs - L phi at one point (solver_2d.py:637, solver_3d.py:642), in the
+// reference's term order.
+// ---------------------------------------------------------------------------
+template <typename T, typename TP>
+__device__ __forceinline__ T resid2(const TP* __restrict__ phi, const TP* __restrict__ rhs,
+                                    const T* __restrict__ cx, const T* __restrict__ cy,
+                                    const T* __restrict__ cc, const T* __restrict__ drr,
+                                    const T* __restrict__ dlr, long p, long sy)
+{
+    T r = cc[p] * (T)phi[p];
+    r = r - cx[p - 1] * (T)phi[p - 1];
+    r = r - cx[p] * (T)phi[p + 1];
+    r = r - cy[p - sy] * (T)phi[p - sy];
+    r = r - cy[p] * (T)phi[p + sy];
+    r = r + (T)rhs[p];
+    if (drr != nullptr) {
+        T off = drr[p - sy - 1] * (T)phi[p - sy - 1] + dlr[p - sy + 1] * (T)phi[p - sy + 1];
+        off = off + dlr[p] * (T)phi[p + sy - 1];
+        off = off + drr[p] * (T)phi[p + sy + 1];
+        r = r - off;
+    }
+    return r;
+}
+
+template <typename T, typename TP>
+__device__ __forceinline__ T resid3(const TP* __restrict__ phi, const TP* __restrict__ rhs,
+                                    const T* __restrict__ cx, const T* __restrict__ cy,
+                                    const T* __restrict__ cz, const T* __restrict__ cc,
+                                    long p, long sy, long sz)
+{
+    T r = cc[p] * (T)phi[p];
+    r = r - cx[p - 1] * (T)phi[p - 1];
+    r = r - cx[p] * (T)phi[p + 1];
+    r = r - cy[p - sy] * (T)phi[p - sy];
+    r = r - cy[p] * (T)phi[p + sy];
+    r = r - cz[p - sz] * (T)phi[p - sz];
+    r = r - cz[p] * (T)phi[p + sz];
+    r = r + (T)rhs[p];
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Defect + partial full-weighting restriction (solver_2d.py:632-647,
+// solver_3d.py:638-652): one thread per coarse unknown; the residuals it needs
+// (2D: centre + 4 corners, 3D: centre + 12 edge points, all on the first RBGS
+// half) are evaluated in registers, never stored.  Coarse boundary nodes and
+// ghosts are not written (zero from allocation).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_restrict2(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
+            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cc,
+            const T* __restrict__ drr, const T* __restrict__ dlr, Geo gf, Geo gc,
+            const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int J = 2 + blockIdx.y * BY + threadIdx.y;
+    const int K = 2 + blockIdx.x * BX + threadIdx.x;
+    if (J > gc.my || K > gc.mx) return;
+    const long sy = gf.sy;
+    const long c = (long)b * gf.sz + (long)(2 * J - 1) * sy + (2 * K - 1);
+    T s = resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c + sy + 1, sy)
+        + resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c + sy - 1, sy);
+    s = s + resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c - sy + 1, sy);
+    s = s + resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c - sy - 1, sy);
+    const T rcen = resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c, sy);
+    rc[(long)b * gc.sz + (long)J * gc.sy + K] = (T)0.0625 * s + (T)0.25 * rcen;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
+            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+            const T* __restrict__ cc, Geo gf, Geo gc)
+{
+    const int I = 2 + blockIdx.z;
+    const int J = 2 + blockIdx.y * BY + threadIdx.y;
+    const int K = 2 + blockIdx.x * BX + threadIdx.x;
+    if (J > gc.my || K > gc.mx) return;
+    const long sy = gf.sy, sz = gf.sz;
+    const long c = (long)(2 * I - 1) * sz + (long)(2 * J - 1) * sy + (2 * K - 1);
+#define JSV_R(o) resid3<T, T>(phi, rhs, cx, cy, cz, cc, c + (o), sy, sz)
+    T e = JSV_R(sy + 1) + JSV_R(sy - 1);
+    e = e + JSV_R(-sy + 1);
+    e = e + JSV_R(-sy - 1);
+    e = e + JSV_R(sz + 1);
+    e = e + JSV_R(sz - 1);
+    e = e + JSV_R(-sz + 1);
+    e = e + JSV_R(-sz - 1);
+    e = e + JSV_R(sz + sy);
+    e = e + JSV_R(sz - sy);
+    e = e + JSV_R(-sz + sy);
+    e = e + JSV_R(-sz - sy);
+    const T rcen = JSV_R(0);
+#undef JSV_R
+    rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * rcen;
+}
+
+// ---------------------------------------------------------------------------
+// Prolongation + add (solver_2d.py:612, 649-668; solver_3d.py:618, 654-677).
+// Fine index i <-> coarse (i+1)/2 for odd i; an even i lies between coarse
+// i/2 and i/2+1.  The interpolations the reference skips when nu2 > 0 (points
+// of the first RBGS half, overwritten before they are read) are skipped alike.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_prolong2(T* __restrict__ phi, const T* __restrict__ pcb, Geo gf, Geo gc, int full,
+           const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    if (y > gf.my || x > gf.mx) return;
+    const T* __restrict__ pc = pcb + (long)b * gc.sz;
+    const long csy = gc.sy;
+    const int ox = x & 1, oy = y & 1;
+    const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Xl = x >> 1, Yl = y >> 1;
+#define C2(J, K) pc[(long)(J) * csy + (K)]
+    T corr;
+    if (ox && oy) {
+        corr = C2(Yo, Xo);
+    } else if (oy) {
+        corr = (T)0.5 * (C2(Yo, Xl + 1) + C2(Yo, Xl));
+    } else if (ox) {
+        corr = (T)0.5 * (C2(Yl + 1, Xo) + C2(Yl, Xo));
+    } else {
+        if (!full) return;
+        T s = C2(Yl + 1, Xl + 1) + C2(Yl, Xl + 1);
+        s = s + C2(Yl + 1, Xl);
+        s = s + C2(Yl, Xl);
+        corr = (T)0.25 * s;
+    }
+#undef C2
+    const long p = (long)b * gf.sz + (long)y * gf.sy + x;
+    phi[p] = phi[p] + corr;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int full)
+{
+    const int z = 2 + blockIdx.z;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    if (y > gf.my || x > gf.mx) return;
+    const long csy = gc.sy, csz = gc.sz;
+    const int ox = x & 1, oy = y & 1, oz = z & 1;
+    const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Zo = (z + 1) >> 1;
+    const int Xl = x >> 1, Yl = y >> 1, Zl = z >> 1;
+#define C3(I, J, K) pc[(long)(I) * csz + (long)(J) * csy + (K)]
+    const int nodd = ox + oy + oz;
+    T corr;
+    if (nodd == 3) {
+        corr = C3(Zo, Yo, Xo);
+    } else if (nodd == 2) {
+        if (!ox)      corr = (T)0.5 * (C3(Zo, Yo, Xl + 1) + C3(Zo, Yo, Xl));
+        else if (!oy) corr = (T)0.5 * (C3(Zo, Yl + 1, Xo) + C3(Zo, Yl, Xo));
+        else          corr = (T)0.5 * (C3(Zl + 1, Yo, Xo) + C3(Zl, Yo, Xo));
+    } else if (nodd == 1) {
+        if (!full) return;
+        T s;
+        if (oz) {
+            s = C3(Zo, Yl + 1, Xl + 1) + C3(Zo, Yl, Xl + 1);
+            s = s + C3(Zo, Yl + 1, Xl);
+            s = s + C3(Zo, Yl, Xl);
+        } else if (oy) {
+            s = C3(Zl + 1, Yo, Xl + 1) + C3(Zl, Yo, Xl + 1);
+            s = s + C3(Zl + 1, Yo, Xl);
+            s = s + C3(Zl, Yo, Xl);
+        } else {
+            s = C3(Zl + 1, Yl + 1, Xo) + C3(Zl, Yl + 1, Xo);
+            s = s + C3(Zl + 1, Yl, Xo);
+            s = s + C3(Zl, Yl, Xo);
+        }
+        corr = (T)0.25 * s;
+    } else {
+        T s = C3(Zl + 1, Yl + 1, Xl + 1) + C3(Zl, Yl + 1, Xl + 1);
+        s = s + C3(Zl + 1, Yl, Xl + 1);
+        s = s + C3(Zl, Yl, Xl + 1);
+        s = s + C3(Zl + 1, Yl + 1, Xl);
+        s = s + C3(Zl, Yl + 1, Xl);
+        s = s + C3(Zl + 1, Yl, Xl);
+        s = s + C3(Zl, Yl, Xl);
+        corr = (T)0.125 * s;
+    }
+#undef C3
+    const long p = (long)z * gf.sz + (long)y * gf.sy + x;
+    phi[p] = phi[p] + corr;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.  Wave64 shuffles, then LDS across the 4 waves of a workgroup.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// max |a - b| over n elements (distance, solver_2d.py:481-494).  Non-negative
+// doubles order like their bit patterns, so a 64-bit atomicMax on the bits is
+// exact and order-independent; a NaN (positive after fabs) compares above
+// +inf, so a NaN anywhere yields NaN as jnp.max does.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_maxabsdiff(const T* __restrict__ a0, const T* __restrict__ b0, long n, unsigned long long* out0)
+{
+    // blockIdx.y selects a segment of n elements (one problem of a batch)
+    __shared__ unsigned long long sh[4];
+    const T* __restrict__ a = a0 + (long)blockIdx.y * n;
+    const T* __restrict__ b = b0 + (long)blockIdx.y * n;
+    unsigned long long* out = out0 + blockIdx.y;
+    unsigned long long best = 0ull;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        double d = fabs((double)a[i] - (double)b[i]);
+        unsigned long long u = (unsigned long long)__double_as_longlong(d);
+        best = u > best ? u : best;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long v = __shfl_xor(best, o, 64);
+        best = v > best ? v : best;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = sh[0];
+        for (int i = 1; i < 4; ++i) m = sh[i] > m ? sh[i] : m;
+        atomicMax(out, m);
+    }
+}
+
+// Residual rhs - L phi over the unknowns of the fine level, in the arithmetic
+// type T of the operator; optionally stores it (as TO) and writes per-block
+// partial sums of r^2 (FP64) for a deterministic two-pass norm.
+template <typename T, typename TP, typename TO>
+__global__ void __launch_bounds__(BX * BY)
+k_residual(const TP* __restrict__ phi, const TP* __restrict__ rhs, const T* __restrict__ cx,
+           const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc,
+           const T* __restrict__ drr, const T* __restrict__ dlr, Geo g, TO* __restrict__ rout,
+           double* __restrict__ partials, int dim)
+{
+    __shared__ double sh[BY];
+    const int z = (dim == 3) ? 2 + (int)blockIdx.z : (int)blockIdx.z;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    double v = 0.0;
+    if (y <= g.my && x <= g.mx) {
+        const long p = (long)z * g.sz + (long)y * g.sy + x;
+        T r;
+        if (dim == 3) r = resid3<T, TP>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);
+        else          r = resid2<T, TP>(phi, rhs, cx, cy, cc, drr, dlr, p, g.sy);
+        if (rout != nullptr) rout[p] = (TO)r;
+        v = (double)r * (double)r;
+    }
+    v = wave_sum(v);
+    if (threadIdx.x == 0) sh[threadIdx.y] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double s = sh[0];
+        for (int i = 1; i < BY; ++i) s += sh[i];
+        partials[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// sum of squares over the unknowns of one array (||rhs||_2^2)
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_norm2_unknowns(const T* __restrict__ a, Geo g, double* __restrict__ partials, int dim)
+{
+    __shared__ double sh[BY];
+    const int z = (dim == 3) ? 2 + (int)blockIdx.z : (int)blockIdx.z;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    double v = 0.0;
+    if (y <= g.my && x <= g.mx) {
+        const double q = (double)a[(long)z * g.sz + (long)y * g.sy + x];
+        v = q * q;
+    }
+    v = wave_sum(v);
+    if (threadIdx.x == 0) sh[threadIdx.y] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double s = sh[0];
+        for (int i = 1; i < BY; ++i) s += sh[i];
+        partials[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// Second pass: workgroup g sums partials [g*n, (g+1)*n) in a fixed order
+// (one segment per batched problem, or one for a whole 3D level).
+__global__ void __launch_bounds__(1024)
+k_sum_partials(const double* __restrict__ partials, long n, double* __restrict__ out)
+{
+    __shared__ double sh[16];
+    double s = 0.0;
+    const double* seg = partials + (long)blockIdx.x * n;
+    for (long i = threadIdx.x; i < n; i += 1024) s += seg[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < 16; ++i) t += sh[i];
+        out[blockIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T* __restrict__ a, long n, T v)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) a[i] = v;
+}
+
+__global__ void __launch_bounds__(64) k_set_u64(unsigned long long* a, unsigned long long v)
+{
+    if (threadIdx.x == 0) a[0] = v;
+}
+
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256) k_cast(const TS* __restrict__ s, TD* __restrict__ d, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) d[i] = (TD)s[i];
+}
+
+// phi64 += (double) e over the unknowns (FP64 refinement correction)
+template <typename TE>
+__global__ void __launch_bounds__(BX * BY)
+k_correct(double* __restrict__ phi, const TE* __restrict__ e, Geo g, int dim,
+          const int* __restrict__ act)
+{
+    if (dim == 2 && act != nullptr && act[blockIdx.z] == 0) return;
+    const int z = (dim == 3) ? 2 + (int)blockIdx.z : (int)blockIdx.z;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    if (y > g.my || x > g.mx) return;
+    const long p = (long)z * g.sz + (long)y * g.sy + x;
+    phi[p] = phi[p] + (double)e[p];
+}
+
+// zero the first/last rim+1 layers of the chosen axes (do_BCs with zero
+// values, boundary_handler_2d.py:19-29); vals given per face where non-null.
+// Axis order as the reference applies it: x faces, then y, then z (3D).
+__global__ void __launch_bounds__(256)
+k_bcs(double* __restrict__ a, Geo g, int dim, int rim, int direction,
+      const double* __restrict__ vxb, const double* __restrict__ vxe,
+      const double* __restrict__ vyb, const double* __restrict__ vye, int phase)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.n) return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = (int)(i / ((long)g.nx * g.ny));
+    const int w = rim + 1;
+    if (dim == 2) {
+        if (phase == 0 && (direction == -1 || direction == 0)) {
+            if (x < w) a[i] = vxb ? vxb[y] : 0.0;
+            else if (x >= g.nx - w) a[i] = vxe ? vxe[y] : 0.0;
+        }
+        if (phase == 1 && (direction == -1 || direction == 1)) {
+            if (y < w) a[i] = vyb ? vyb[x] : 0.0;
+            else if (y >= g.ny - w) a[i] = vye ? vye[x] : 0.0;
+        }
+    } else {
+        if (phase == 0 && (direction == -1 || direction == 0)) {
+            if (x < w) a[i] = vxb ? vxb[(long)z * g.ny + y] : 0.0;
+            else if (x >= g.nx - w) a[i] = vxe ? vxe[(long)z * g.ny + y] : 0.0;
+        }
+        if (phase == 1 && (direction == -1 || direction == 1)) {
+            if (y < w) a[i] = vyb ? vyb[(long)z * g.nx + x] : 0.0;
+            else if (y >= g.ny - w) a[i] = vye ? vye[(long)z * g.nx + x] : 0.0;
+        }
+        if (phase == 2 && (direction == -1 || direction == 2)) {
+            // the reference's z faces read vals_yb / vals_ye, indexed [y, x]
+            if (z < w) a[i] = vyb ? vyb[(long)y * g.nx + x] : 0.0;
+            else if (z >= g.nz - w) a[i] = vye ? vye[(long)y * g.nx + x] : 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient setup (compute_discretization and helpers).
+// ---------------------------------------------------------------------------
+struct Src {              // a fine-level input array sampled at level l
+    const double* a;      // null -> use scalar
+    long sy0, sz0;        // fine strides
+    int stride;           // 2^l
+    int lvl0;             // level 0 reads the array as is (ghosts included)
+};
+
+__device__ __forceinline__ double sample(const Src& s, const Geo& g, int z, int y, int x, int dim)
+{
+    if (s.lvl0) return s.a[(long)z * s.sz0 + (long)y * s.sy0 + x];
+    // injection chain of the reference = strided sample of the fine array on
+    // the [1:-1] window, zero on ghosts (solver_2d.py:353, 381)
+    if (y < 1 || y > g.my + 1 || x < 1 || x > g.mx + 1) return 0.0;
+    if (dim == 3 && (z < 1 || z > g.mz + 1)) return 0.0;
+    const long zz = (dim == 3) ? (long)(z - 1) * s.stride + 1 : 0;
+    return s.a[zz * s.sz0 + ((long)(y - 1) * s.stride + 1) * s.sy0 + (long)(x - 1) * s.stride + 1];
+}
+
+// 2D (solver_2d.py:341-361, 395-421): raw injection at every level, then face
+// averages at every level including level 0.
+__global__ void __launch_bounds__(256)
+k_setup2d(double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cc,
+          double* __restrict__ drr, double* __restrict__ dlr, Geo g,
+          Src Dx, Src Dy, Src Lam, Src Dxy, double sqx, double sqy, double cxy,
+          double lam_s, double dxs, double dys, int spatial)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.n) return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int bz = (int)(i / g.sz);            // problem of the batch
+    Dx.a = Dx.a ? Dx.a + (long)bz * Dx.sz0 : nullptr;
+    Dy.a = Dy.a ? Dy.a + (long)bz * Dy.sz0 : nullptr;
+    Lam.a = Lam.a ? Lam.a + (long)bz * Lam.sz0 : nullptr;
+    Dxy.a = Dxy.a ? Dxy.a + (long)bz * Dxy.sz0 : nullptr;
+    const bool win = (y >= 1 && y <= g.my + 1 && x >= 1 && x <= g.mx + 1);
+    const double lam = Lam.a ? (win ? sample(Lam, g, 0, y, x, 2) : 0.0) : lam_s;
+    if (!spatial) {
+        const double ax = dxs * sqx, ay = dys * sqy;
+        cx[i] = ax;
+        cy[i] = ay;
+        cc[i] = win ? 2.0 * (ax + ay) + lam : 0.0;
+        return;
+    }
+    auto avgx = [&](int yy, int xx) -> double {
+        if (yy >= 1 && yy <= g.my + 1 && xx >= 0 && xx <= g.mx + 1)
+            return 0.5 * (sample(Dx, g, 0, yy, xx + 1, 2) + sample(Dx, g, 0, yy, xx, 2));
+        return sample(Dx, g, 0, yy, xx, 2);
+    };
+    auto avgy = [&](int yy, int xx) -> double {
+        if (yy >= 0 && yy <= g.my + 1 && xx >= 1 && xx <= g.mx + 1)
+            return 0.5 * (sample(Dy, g, 0, yy + 1, xx, 2) + sample(Dy, g, 0, yy, xx, 2));
+        return sample(Dy, g, 0, yy, xx, 2);
+    };
+    const double rc = avgx(y, x) * sqx;
+    const double cr = avgy(y, x) * sqy;
+    cx[i] = rc;
+    cy[i] = cr;
+    if (win) {
+        const double lc = avgx(y, x - 1) * sqx;
+        const double cl = avgy(y - 1, x) * sqy;
+        cc[i] = lc + rc + cl + cr + lam;
+    } else {
+        cc[i] = 0.0;
+    }
+    if (drr != nullptr) {
+        if (win) {
+            const double q_xp = sample(Dxy, g, 0, y, x + 1, 2);
+            const double q_yp = sample(Dxy, g, 0, y + 1, x, 2);
+            const double q_xm = sample(Dxy, g, 0, y, x - 1, 2);
+            drr[i] = (q_xp + q_yp) * cxy;
+            dlr[i] = -(q_xm + q_yp) * cxy;
+        } else {
+            drr[i] = 0.0;
+            dlr[i] = 0.0;
+        }
+    }
+}
+
+// 3D level l >= 1 face-averaged D (solver_3d.py:363-370): injection from the
+// previous level's (already averaged, except at level 0) array, then the
+// average along `axis` over the reference's window.
+__global__ void __launch_bounds__(256)
+k_avg3(double* __restrict__ dst, const double* __restrict__ src, Geo g, Geo gs, int axis)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.n) return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = (int)(i / ((long)g.nx * g.ny));
+    auto inj = [&](int zz, int yy, int xx) -> double {
+        if (zz < 1 || zz > g.mz + 1 || yy < 1 || yy > g.my + 1 || xx < 1 || xx > g.mx + 1) return 0.0;
+        return src[(long)(2 * zz - 1) * gs.sz + (long)(2 * yy - 1) * gs.sy + (2 * xx - 1)];
+    };
+    bool in;
+    int dz = 0, dy = 0, dx = 0;
+    if (axis == 0) {        // x
+        in = (z >= 1 && z <= g.mz + 1 && y >= 1 && y <= g.my + 1 && x <= g.mx + 1);
+        dx = 1;
+    } else if (axis == 1) { // y
+        in = (z >= 1 && z <= g.mz + 1 && y <= g.my + 1 && x >= 1 && x <= g.mx + 1);
+        dy = 1;
+    } else {                // z
+        in = (z <= g.mz + 1 && y >= 1 && y <= g.my + 1 && x >= 1 && x <= g.mx + 1);
+        dz = 1;
+    }
+    dst[i] = in ? 0.5 * (inj(z + dz, y + dy, x + dx) + inj(z, y, x)) : inj(z, y, x);
+}
+
+// 3D coefficients of one level from its face arrays (level 0: the raw input,
+// never averaged; solver_3d.py:404-429)
+__global__ void __launch_bounds__(256)
+k_setup3d(double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
+          double* __restrict__ cc, Geo g, const double* __restrict__ ax,
+          const double* __restrict__ ay, const double* __restrict__ az, Src Lam,
+          double sqx, double sqy, double sqz, double lam_s, double dxs, double dys, double dzs,
+          int spatial)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.n) return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = (int)(i / ((long)g.nx * g.ny));
+    const bool win = (z >= 1 && z <= g.mz + 1 && y >= 1 && y <= g.my + 1 && x >= 1 && x <= g.mx + 1);
+    const double lam = Lam.a ? (win ? sample(Lam, g, z, y, x, 3) : 0.0) : lam_s;
+    if (!spatial) {
+        const double axs = dxs * sqx, ays = dys * sqy, azs = dzs * sqz;
+        cx[i] = axs;
+        cy[i] = ays;
+        cz[i] = azs;
+        cc[i] = win ? 2.0 * (axs + ays + azs) + lam : 0.0;
+        return;
+    }
+    const double rcc = ax[i] * sqx, crc = ay[i] * sqy, ccr = az[i] * sqz;
+    cx[i] = rcc;
+    cy[i] = crc;
+    cz[i] = ccr;
+    if (win) {
+        const double lcc = ax[i - 1] * sqx;
+        const double clc = ay[i - g.sy] * sqy;
+        const double ccl = az[i - g.sz] * sqz;
+        cc[i] = lcc + rcc + clc + crc + ccl + ccr + lam;
+    } else {
+        cc[i] = 0.0;
+    }
+}
+
+}  // namespace jsv

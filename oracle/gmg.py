@@ -17,11 +17,11 @@ multiply-adds (the reference's own tests accept 1e-9 relative).
 
 Pinning (what checks each function, see tests/test_oracle_*.py):
   * level_layout / grids: the reference's error-message tests and grid tests.
-  * setup_2d (scalar and spatial D, D_xy), smooth, restrict, prolong, cycle,
+  * build_operator 2D (scalar and spatial D, D_xy), smooth, restrict, prolong, cycle,
     solve (V/F/W, fixed/dynamic, skip_pre): the exact iteration counts and
     RMSE goldens of tests/test_solver_2d.py and examples/benchmark.py tables.
-  * setup_3d scalar path and the 3D cycle: goldens of tests/test_solver_3d.py.
-  * setup_3d spatial path (no golden exists): closed forms -- constant D
+  * build_operator 3D scalar path and the 3D cycle: goldens of tests/test_solver_3d.py.
+  * build_operator 3D spatial path (no golden exists): closed forms -- constant D
     arrays reproduce the scalar operator at level 0 and 1 and give the
     (1/2 + 2^-l) D upper faces at level l >= 2; the brute-force sparse direct
     solve of the level-0 operator equals the converged multigrid solution;
@@ -77,9 +77,14 @@ class OracleGrid:
 # ----------------------------------------------------------------------------
 # do_BCs (jsolver/boundary_handler_2d.py:3-29, boundary_handler_3d.py:3-35)
 # ----------------------------------------------------------------------------
+def _ft(*arrs):
+    """float64, or complex128 when any input is complex (complex-step JVPs)."""
+    return np.result_type(np.float64, *[np.asarray(a).dtype for a in arrs])
+
+
 def zero_rim(a: np.ndarray, rim: int = 1) -> np.ndarray:
     """Return a copy with the first/last rim+1 layers of every axis zeroed."""
-    out = np.array(a, dtype=np.float64, copy=True)
+    out = np.array(a, dtype=_ft(a), copy=True)
     w = rim + 1
     for ax in range(out.ndim):
         idx = [slice(None)] * out.ndim
@@ -190,11 +195,12 @@ class Operator:
     levels: int
     lv: List[Level] = field(default_factory=list)
     off_diag: bool = False
+    dtype: object = np.float64     # complex128 when a coefficient input is complex
 
 
 def _inject(prev: np.ndarray, shape) -> np.ndarray:
     """zeros(shape) with [1:-1]^d <- prev[1::2]^d (solver_2d.py:353, solver_3d.py:364)."""
-    out = np.zeros(shape)
+    out = np.zeros(shape, dtype=_ft(prev))
     d = len(shape)
     out[(slice(1, -1),) * d] = prev[(slice(1, None, 2),) * d]
     return out
@@ -232,7 +238,8 @@ def build_operator(dim, ms, idel, lam, D, D_xy=None) -> Operator:
     sq = [(np.float64(i) * 0.5 ** np.arange(levels)) ** 2 for i in idel]
     # array axis a corresponds to physical axis (dim-1-a)
     sq_ax = [sq[dim - 1 - a] for a in range(dim)]
-    op = Operator(dim=dim, levels=levels)
+    op = Operator(dim=dim, levels=levels,
+                  dtype=_ft(lam, *D, *([] if D_xy is None else [D_xy])))
     scalar_lam = _is_scalar(lam)
     spatial = not all(_is_scalar(x) for x in D)
     op.off_diag = spatial and D_xy is not None
@@ -240,13 +247,13 @@ def build_operator(dim, ms, idel, lam, D, D_xy=None) -> Operator:
     lam_l = None
     if not scalar_lam:
         # compute_lambda: 2d:365-383 / 3d:374-392
-        lam_l = [np.asarray(lam, dtype=np.float64)]
+        lam_l = [np.asarray(lam, dtype=_ft(lam))]
         for l in range(1, levels):
             lam_l.append(_inject(lam_l[-1], shapes[l]))
 
     if spatial:
         # physical axis order D = (D_xx, D_yy[, D_zz]); array axis a <-> D[dim-1-a]
-        Dl = [[np.asarray(D[dim - 1 - a], dtype=np.float64)] for a in range(dim)]
+        Dl = [[np.asarray(D[dim - 1 - a], dtype=_ft(D[dim - 1 - a]))] for a in range(dim)]
         if dim == 2:
             # inject all levels from raw values first, then average every level
             for a in range(dim):
@@ -278,7 +285,7 @@ def build_operator(dim, ms, idel, lam, D, D_xy=None) -> Operator:
             lev = Level(shape=shapes[l], a_minus=minus, a_plus=plus, a_c=acc)
             op.lv.append(lev)
         if op.off_diag:
-            Dxy = [np.asarray(D_xy, dtype=np.float64)]
+            Dxy = [np.asarray(D_xy, dtype=_ft(D_xy))]
             for l in range(1, levels):
                 Dxy.append(_inject(Dxy[-1], shapes[l]))
             idx = np.float64(idel[0]) * 0.5 ** np.arange(levels)
@@ -326,7 +333,7 @@ def smooth(op: Operator, level: int, phi: np.ndarray, rhs: np.ndarray) -> np.nda
     """One red-black Gauss-Seidel sweep over all sub-colours, in place on a copy."""
     lv = op.lv[level]
     d = op.dim
-    phi = zero_rim(phi)
+    phi = zero_rim(np.asarray(phi, dtype=np.result_type(_ft(phi, rhs), op.dtype)))
     orders = SUBCOLORS_2D if d == 2 else SUBCOLORS_3D
     for col in orders:
         # col is (lx, ly[, lz]); array axis a uses col[d-1-a]
@@ -399,11 +406,12 @@ def residual_window(op: Operator, level: int, phi: np.ndarray, rhs: np.ndarray) 
 
 def restrict(op: Operator, level: int, phi: np.ndarray, rhs: np.ndarray) -> np.ndarray:
     d = op.dim
-    full = np.zeros(rhs.shape)
-    full[(slice(1, -1),) * d] = residual_window(op, level, phi, rhs)
+    rw = residual_window(op, level, phi, rhs)
+    full = np.zeros(rhs.shape, dtype=rw.dtype)
+    full[(slice(1, -1),) * d] = rw
     full = zero_rim(full)
     cshape = op.lv[level + 1].shape
-    rc = np.zeros(cshape)
+    rc = np.zeros(cshape, dtype=full.dtype)
     P, M, C = slice(2, None, 2), slice(None, -2, 2), slice(1, -1, 2)   # +1, -1, centre
     if d == 2:
         rc[1:-1, 1:-1] = 0.0625 * (full[P, P] + full[P, M] + full[M, P] + full[M, M]) + 0.25 * full[C, C]
@@ -421,7 +429,7 @@ def restrict(op: Operator, level: int, phi: np.ndarray, rhs: np.ndarray) -> np.n
 def prolong(op: Operator, level: int, phic: np.ndarray, nu2: int) -> np.ndarray:
     d = op.dim
     shape = op.lv[level].shape
-    c = np.zeros(shape)
+    c = np.zeros(shape, dtype=_ft(phic))
     O = slice(1, -1, 2)         # odd indices 1..m+1 (coarse points)
     E = slice(2, -1, 2)         # even indices 2..m
     Pp = slice(3, None, 2)      # even + 1
@@ -469,7 +477,7 @@ def cycle(op: Operator, cfg: Config, phi, rhs, fcyc=False, level=0, skip_pre=Fal
         for _ in range(cfg.nu1):
             phi = smooth(op, level, phi, rhs)
     rc = restrict(op, level, phi, rhs)
-    pc = np.zeros(op.lv[level + 1].shape)
+    pc = np.zeros(op.lv[level + 1].shape, dtype=rc.dtype)
     if fcyc:
         for _ in range(cfg.gamma):
             pc = cycle(op, cfg, pc, rc, True, level + 1)
@@ -483,8 +491,8 @@ def cycle(op: Operator, cfg: Config, phi, rhs, fcyc=False, level=0, skip_pre=Fal
 
 def solve(op: Operator, cfg: Config, phi, rhs, history=None):
     """Returns (phi, count).  history: optional list receiving phi after each cycle."""
-    phi = np.asarray(phi, dtype=np.float64)
-    rhs = np.asarray(rhs, dtype=np.float64)
+    phi = np.asarray(phi, dtype=_ft(phi))
+    rhs = np.asarray(rhs, dtype=_ft(rhs))
     if cfg.skip_pre:
         for _ in range(cfg.nu1):
             phi = smooth(op, 0, phi, rhs)
@@ -517,7 +525,7 @@ def apply_operator(op: Operator, phi: np.ndarray, level: int = 0) -> np.ndarray:
     d = op.dim
     z = np.zeros(phi.shape)
     r = residual_window(op, level, phi, z)   # = A_c phi - sum A_nb phi_nb - off
-    out = np.zeros(phi.shape)
+    out = np.zeros(phi.shape, dtype=r.dtype)
     inner = (slice(1, -1),) * d
     out[inner] = -r
     return zero_rim(out)
@@ -526,8 +534,9 @@ def apply_operator(op: Operator, phi: np.ndarray, level: int = 0) -> np.ndarray:
 def residual(op: Operator, phi: np.ndarray, rhs: np.ndarray, level: int = 0) -> np.ndarray:
     """rhs - L phi on the unknowns (zero on boundary nodes and ghosts)."""
     d = op.dim
-    out = np.zeros(phi.shape)
-    out[(slice(1, -1),) * d] = residual_window(op, level, zero_rim(phi), rhs)
+    rw = residual_window(op, level, zero_rim(phi), rhs)
+    out = np.zeros(phi.shape, dtype=rw.dtype)
+    out[(slice(1, -1),) * d] = rw
     return zero_rim(out)
 
 

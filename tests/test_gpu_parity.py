@@ -1,0 +1,349 @@
+"""GPU parity: every HIP operation of the multigrid path against the CPU oracle.
+
+FP64 kernels evaluate the reference's formulas term by term with no FMA
+contraction, so per-operation results are compared BIT-EXACTLY with the
+oracle (tolerance 0).  Whole solves are compared the same way, plus the
+reference's own goldens (exact iteration counts, RMSE within 1e-9 rel).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_helpers import DevPlan, cfg, oracle_operator, rand_problem
+from oracle import gmg
+from problems import problem_2d, problem_3d
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = json.load(open(os.path.join(HERE, "golden", "reference_goldens.json")))
+
+CASES = [
+    # id, dim, ms (x,y[,z]), spatial, lam_array, dxy
+    ("2d_spatial_16", 2, (16, 16), True, True, False),
+    ("2d_spatial_16x32", 2, (16, 32), True, True, False),
+    ("2d_spatial_32x8", 2, (32, 8), True, False, False),
+    ("2d_scalarD_lamarr", 2, (16, 16), False, True, False),
+    ("2d_scalar", 2, (16, 16), False, False, False),
+    ("2d_dxy_16", 2, (16, 16), True, True, True),
+    ("2d_dxy_8x16", 2, (8, 16), True, True, True),
+    ("3d_spatial_8", 3, (8, 8, 8), True, True, False),
+    ("3d_spatial_8x16x8", 3, (8, 16, 8), True, True, False),
+    ("3d_spatial_16x8x32", 3, (16, 8, 32), True, False, False),
+    ("3d_scalar_lamarr", 3, (8, 8, 8), False, True, False),
+]
+
+
+def _setup(case, seed=1):
+    _, dim, ms, spatial, lam_array, dxy = case
+    phi, rhs, lam, D, Dxy = rand_problem(dim, ms, seed, spatial, lam_array, dxy)
+    op = oracle_operator(dim, ms, lam, D, Dxy)
+    dp = DevPlan(dim, ms, lam, D, Dxy)
+    return dp, op, phi, rhs
+
+
+def _level_arrays(op, level, seed):
+    rng = np.random.default_rng(seed)
+    shape = op.lv[level].shape
+    return gmg.zero_rim(rng.standard_normal(shape)), rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_smooth_bitexact(case):
+    dp, op, _, _ = _setup(case)
+    for level in range(op.levels - 1):
+        phi, rhs = _level_arrays(op, level, 10 + level)
+        dp.upload(level, 0, phi)
+        dp.upload(level, 1, rhs)
+        dp.smooth(level, 2)
+        got = dp.download(level, 0)
+        want = gmg.smooth(op, level, gmg.smooth(op, level, phi, rhs), rhs)
+        np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
+
+
+@pytest.mark.parametrize("slab_bytes", [1, 3 * 8 * 6 * 35 * 19])
+def test_smooth_3d_slab_schedule_bitexact(slab_bytes):
+    """The z-slab launch schedule of the 3D smoother (red on slab k, black on
+    slab k-1) gives the same bits as one red and one black launch."""
+    from jsolver_amd import _lib
+    case = ("3d_spatial_16x32x32", 3, (16, 32, 32), True, True, False)
+    dp, op, _, _ = _setup(case)
+    _lib.check(_lib.load().jsv_plan_set_slab_bytes(dp.plan.h, slab_bytes))
+    for level in range(2):
+        phi, rhs = _level_arrays(op, level, 60 + level)
+        dp.upload(level, 0, phi)
+        dp.upload(level, 1, rhs)
+        dp.smooth(level, 2)
+        got = dp.download(level, 0)
+        want = gmg.smooth(op, level, gmg.smooth(op, level, phi, rhs), rhs)
+        np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_restrict_bitexact(case):
+    dp, op, _, _ = _setup(case)
+    for level in range(op.levels - 2):
+        phi, rhs = _level_arrays(op, level, 20 + level)
+        dp.upload(level, 0, phi)
+        dp.upload(level, 1, rhs)
+        dp.upload(level + 1, 1, np.zeros(op.lv[level + 1].shape))
+        dp.restrict(level)
+        got = dp.download(level + 1, 1)
+        want = gmg.restrict(op, level, phi, rhs)
+        np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
+
+
+@pytest.mark.parametrize("nu2", [1, 0])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_prolong_bitexact(case, nu2):
+    dp, op, _, _ = _setup(case)
+    for level in range(op.levels - 2):
+        phi, _ = _level_arrays(op, level, 30 + level)
+        pc, _ = _level_arrays(op, level + 1, 40 + level)
+        dp.upload(level, 0, phi)
+        dp.upload(level + 1, 0, pc)
+        dp.prolong(level, nu2)
+        got = dp.download(level, 0)
+        want = phi + gmg.prolong(op, level, pc, nu2)
+        np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
+
+
+CYCLES = [
+    ("V", dict()),
+    ("W", dict(gamma=2)),
+    ("F", dict(fcycle=True)),
+    ("V_skip", dict(skip_pre=True)),
+    ("V_nu2_0", dict(nu1=2, nu2=0)),
+    ("W_nu1_2_nu2_3", dict(gamma=2, nu1=2, nu2=3)),
+]
+
+
+@pytest.mark.parametrize("cyc", CYCLES, ids=[c[0] for c in CYCLES])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_cycle_bitexact(case, cyc):
+    dp, op, phi, rhs = _setup(case, seed=5)
+    kw = cyc[1]
+    c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+    phi_raw = phi.copy()                      # non-zero rim: the cycle must zero it
+    dp.upload(0, 0, phi_raw)
+    dp.upload(0, 1, rhs)
+    dp.cycle(cfg(**kw), skip_pre=kw.get("skip_pre", False))
+    got = dp.download(0, 0)
+    want = gmg.cycle(op, c, gmg.zero_rim(phi_raw), rhs, c.fcycle, 0, c.skip_pre)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_residual_matches_oracle(case):
+    dp, op, phi, rhs = _setup(case, seed=7)
+    got, n2 = dp.residual(phi, rhs)
+    want = gmg.residual(op, phi, rhs)
+    np.testing.assert_array_equal(got, want)
+    assert n2 == pytest.approx(float(np.sum(want * want)), rel=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# whole solves through the public API
+# ---------------------------------------------------------------------------
+from jsolver_amd.grid_1d import grid_1D  # noqa: E402
+from jsolver_amd.solver_2d import (solve_2d, solve_2d_fcycle_simple, solve_2d_fixed_simple,  # noqa: E402
+                                   solve_2d_simple, solve_2d_skip_pre_simple,
+                                   solve_2d_wcycle_simple)
+from jsolver_amd.solver_3d import solve_3d, solve_3d_simple  # noqa: E402
+
+REL, ABS = G["tolerance"]["rel"], G["tolerance"]["abs"]
+SMALL_2D = [g for g in G["tests_2d"] if g["mx"] <= 64 and g["my"] <= 128]
+
+
+@pytest.mark.parametrize("g", SMALL_2D, ids=lambda g: f"p{g['problem']}_{g['mx']}x{g['my']}")
+def test_solve_2d_goldens(g):
+    gx, gy = grid_1D(g["mx"]), grid_1D(g["my"])
+    ana, rhs, lam, dx, dy, dxy = problem_2d(g["problem"], gx.centers, gy.centers)
+    phi0 = np.zeros(rhs.shape)
+    out, count = solve_2d_simple(gx, gy, phi0, rhs, lam, dx, dy, dxy)
+    res = out.cpu().numpy()
+    assert count == g["iterations"]
+    assert gmg.rmse(ana, res) == pytest.approx(g["rmse"], abs=ABS, rel=REL)
+    ogx, ogy = gmg.OracleGrid(g["mx"]), gmg.OracleGrid(g["my"])
+    want, wc = gmg.solve_2d(ogx, ogy, phi0, rhs, lam, dx, dy, dxy)
+    assert wc == count
+    np.testing.assert_array_equal(res, want)
+
+
+@pytest.mark.parametrize("g", G["tests_2d_p1_variants"], ids=lambda g: g["kind"])
+def test_solve_2d_p1_variants(g):
+    gx, gy = grid_1D(128), grid_1D(128)
+    ana, rhs, lam, dx, dy, _ = problem_2d(1, gx.centers, gy.centers)
+    phi0 = np.zeros(rhs.shape)
+    fn = {"fcycle": lambda: solve_2d_fcycle_simple(gx, gy, phi0, rhs, lam, dx, dy),
+          "wcycle": lambda: solve_2d(gx, gy, phi0, rhs, lam, dx, dy, gamma=2),
+          "fixed13": lambda: solve_2d_fixed_simple(gx, gy, 13, phi0, rhs, lam, dx, dy),
+          "fixed13_fcycle": lambda: solve_2d(gx, gy, phi0, rhs, lam, dx, dy, fixed=True,
+                                             max_iters_outer=13, fcycle=True)}[g["kind"]]
+    out, count = fn()
+    if g["exact"]:
+        assert count == g["iterations"]
+    else:
+        assert count <= g["iterations"]
+    assert gmg.rmse(ana, out.cpu().numpy()) == pytest.approx(g["rmse"], rel=g["rel"])
+
+
+@pytest.mark.parametrize("g", G["tests_3d"], ids=lambda g: f"p{g['problem']}_{g['mx']}x{g['my']}x{g['mz']}")
+def test_solve_3d_goldens(g):
+    gx, gy, gz = grid_1D(g["mx"]), grid_1D(g["my"]), grid_1D(g["mz"])
+    ana, rhs, lam, dx, dy, dz = problem_3d(g["problem"], gx.centers, gy.centers, gz.centers)
+    phi0 = np.zeros(rhs.shape)
+    out, count = solve_3d_simple(gx, gy, gz, phi0, rhs, lam, dx, dy, dz)
+    assert count == g["iterations"]
+    assert gmg.rmse(ana, out.cpu().numpy()) == pytest.approx(g["rmse"], abs=ABS, rel=REL)
+
+
+TABLE_KW = {"reference_v": dict(), "reference_vskip": dict(skip_pre=True),
+            "reference_f": dict(fcycle=True), "reference_fskip": dict(fcycle=True, skip_pre=True),
+            "reference_w": dict(gamma=2)}
+
+
+def _table_rows(res_set):
+    out = []
+    for name in TABLE_KW:
+        for p, res, it, val in G["tables_2d"][name]["rows"]:
+            if res in res_set and np.isfinite(val) and not (name == "reference_v" and p == 4 and res == 64):
+                out.append(pytest.param(name, p, res, it, val, id=f"{name}-p{p}-{res}"))
+    return out
+
+
+# The tables were produced on the reference author's platform (a JAX/XLA
+# evaluation with its own operation fusion); at 256^2 one row differs from the
+# IEEE evaluation here by 1.06e-9 relative RMSE with the same iteration count,
+# so table rows use 5e-9 (the reference's tests themselves use 1e-9, which the
+# test-suite goldens above meet).
+TABLE_REL = 5e-9
+
+
+@pytest.mark.parametrize("name,p,res,it,val", _table_rows({8, 16, 32, 256}))
+def test_solve_2d_table_goldens(name, p, res, it, val):
+    gx = grid_1D(res)
+    ana, rhs, lam, dx, dy, dxy = problem_2d(p, gx.centers, gx.centers)
+    out, count = solve_2d(gx, gx, np.zeros(rhs.shape), rhs, lam, dx, dy, D_xy=dxy, **TABLE_KW[name])
+    assert count == it
+    assert gmg.rmse(ana, out.cpu().numpy()) == pytest.approx(val, abs=ABS, rel=TABLE_REL)
+
+
+def test_solve_3d_spatial_matches_oracle_per_cycle():
+    """3D spatially varying D (no reference golden): every cycle's iterate
+    equals the oracle's bit for bit, and the dynamic count matches."""
+    ms = (16, 16, 32)
+    phi, rhs, lam, D, _ = rand_problem(3, ms, seed=11)
+    gr = [grid_1D(m) for m in ms]
+    ogr = [gmg.OracleGrid(m) for m in ms]
+    phi0 = np.zeros(rhs.shape)
+    hist_o = []
+    want, wc = gmg.solve_3d(*ogr, phi0, rhs, lam, *D, history=hist_o, fixed=True, max_iters_outer=6)
+    for k in range(1, 7):
+        out, c = solve_3d(*gr, phi0, rhs, lam, *D, fixed=True, max_iters_outer=k)
+        np.testing.assert_array_equal(out.cpu().numpy(), hist_o[k - 1], err_msg=f"cycle {k}")
+    want, wc = gmg.solve_3d(*ogr, phi0, rhs, lam, *D)
+    out, count = solve_3d(*gr, phi0, rhs, lam, *D)
+    assert count == wc
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
+def test_solve_nonzero_rim_and_warm_start():
+    """A phi with non-zero boundary/ghost values: the first distance includes
+    them (solver_2d.py:471-473) and the result matches the oracle."""
+    ms = (32, 32)
+    phi, rhs, lam, D, _ = rand_problem(2, ms, seed=3)
+    g = [grid_1D(m) for m in ms]
+    og = [gmg.OracleGrid(m) for m in ms]
+    for kw in (dict(), dict(skip_pre=True), dict(gamma=2, fcycle=True)):
+        out, count = solve_2d(*g, phi, rhs, lam, *D, **kw)
+        want, wc = gmg.solve_2d(*og, phi, rhs, lam, *D, **kw)
+        assert count == wc
+        np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
+def test_inputs_not_mutated_and_torch_inputs():
+    ms = (16, 16)
+    phi, rhs, lam, D, _ = rand_problem(2, ms, seed=4)
+    g = [grid_1D(m) for m in ms]
+    t = [torch.from_numpy(a.copy()).cuda() for a in (phi, rhs, lam, D[0], D[1])]
+    keep = [x.clone() for x in t]
+    out, count = solve_2d(*g, *t)
+    for a, b in zip(t, keep):
+        assert torch.equal(a, b)
+    out2, count2 = solve_2d(*g, phi, rhs, lam, *D)
+    assert count == count2 and torch.equal(out, out2)
+
+
+def test_nan_input_stops_after_one_cycle():
+    """NaN in rhs: the reference's `dist > eps` is False for NaN, so the loop
+    ends after one cycle with a NaN phi (solver_2d.py:465-477)."""
+    ms = (16, 16)
+    phi, rhs, lam, D, _ = rand_problem(2, ms, seed=8)
+    rhs[5, 7] = np.nan
+    g = [grid_1D(m) for m in ms]
+    out, count, info = solve_2d(*g, np.zeros(rhs.shape), rhs, lam, *D, return_info=True)
+    assert count == 1
+    assert info["nonfinite"]
+    assert torch.isnan(out).any()
+
+
+# ---------------------------------------------------------------------------
+# do_BCs
+# ---------------------------------------------------------------------------
+def test_do_bcs_2d():
+    from jsolver_amd.boundary_handler_2d import do_BCs
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((19, 23))
+    for kw in (dict(), dict(rim=2), dict(direction=0), dict(direction=1),
+               dict(vals_xb=rng.standard_normal(19), vals_xe=rng.standard_normal(19),
+                    vals_yb=rng.standard_normal(23), vals_ye=rng.standard_normal(23)),
+               dict(direction=1, rim=0, vals_yb=np.full(23, 2.0))):
+        np.testing.assert_array_equal(do_BCs(a, **kw).cpu().numpy(), gmg.bcs_2d(a, **kw))
+    # the reference's own test: ones(131, 131) -> four zero rims
+    z = do_BCs(np.ones((131, 131))).cpu().numpy()
+    assert (z[:2] == 0).all() and (z[-2:] == 0).all() and (z[:, :2] == 0).all() and (z[:, -2:] == 0).all()
+    assert (z[2:-2, 2:-2] == 1).all()
+
+
+def test_do_bcs_3d():
+    from jsolver_amd.boundary_handler_3d import do_BCs
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((9, 9, 12))
+    for kw in (dict(), dict(rim=2), dict(direction=0), dict(direction=1), dict(direction=2),
+               dict(vals_xb=rng.standard_normal((9, 9)), vals_xe=rng.standard_normal((9, 9)),
+                    vals_yb=rng.standard_normal((9, 12)), vals_ye=rng.standard_normal((9, 12)))):
+        np.testing.assert_array_equal(do_BCs(a, **kw).cpu().numpy(), gmg.bcs_3d(a, **kw))
+
+
+# ---------------------------------------------------------------------------
+# batched 2D solves: every problem equals its own solve (bit for bit)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kw", [dict(), dict(stop="rel_residual", rtol=1e-9), dict(gamma=2, fcycle=True),
+                                dict(precision="mixed", stop="rel_residual", rtol=1e-10),
+                                dict(fixed=True, max_iters_outer=5)],
+                         ids=["distance", "relres", "W_F", "mixed", "fixed"])
+def test_batched_equals_single(kw):
+    from jsolver_amd.solver_2d import solve_batched_2d
+    ms = (32, 32)
+    g = [grid_1D(m) for m in ms]
+    B = 5
+    probs = [rand_problem(2, ms, seed=100 + b) for b in range(B)]
+    # spread the coefficient scales so problems stop after different counts
+    for b, pr in enumerate(probs):
+        pr[3][1] *= (1.0 + 3 * b)
+    phi0 = np.stack([np.zeros(pr[1].shape) for pr in probs])
+    rhs = np.stack([pr[1] for pr in probs])
+    lam = np.stack([pr[2] for pr in probs])
+    dx = np.stack([pr[3][0] for pr in probs])
+    dy = np.stack([pr[3][1] for pr in probs])
+    out, counts, _ = solve_batched_2d(*g, phi0, rhs, lam, dx, dy, **kw)
+    singles = [solve_2d(*g, phi0[b], rhs[b], lam[b], dx[b], dy[b], **kw) for b in range(B)]
+    for b in range(B):
+        assert counts[b] == singles[b][1], b
+        np.testing.assert_array_equal(out[b].cpu().numpy(), singles[b][0].cpu().numpy())
+    if not kw.get("fixed"):
+        assert len(set(counts.tolist())) > 1      # the problems really stopped at different cycles

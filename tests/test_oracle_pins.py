@@ -1,0 +1,245 @@
+"""Pins for the oracle where the reference holds no golden vector.
+
+* 3D spatially varying D (solver_3d.py:338-372): closed form of the averaging
+  quirk for constant D arrays; brute-force sparse direct solve of the level-0
+  operator on 8^3 equals the converged multigrid solution.
+* 2D: constant D arrays reproduce the scalar path; brute force on 8^2 and
+  16x8 with and without D_xy.
+* Discrete sine eigenvectors are reproduced exactly (closed form).
+* The operator is symmetric without D_xy; ghost values of D, lambda and rhs
+  have no effect.
+Each check is chosen so a plausible slip (a dropped face, a shifted index, a
+missing average, a sign) fails it.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import gmg
+
+
+def _grids(ms):
+    return [gmg.OracleGrid(m) for m in ms]
+
+
+def _idels(ms):
+    return tuple(g.idel for g in _grids(ms))
+
+
+# ---------------------------------------------------------------------------
+# independent assembly of the level-0 operator from the documented face rules
+# ---------------------------------------------------------------------------
+def assemble_level0(dim, ms, lam, D, Dxy=None):
+    """Sparse matrix of L (sum A_nb phi_nb - A_c phi) on the unknowns 2..m.
+    2D faces: 0.5 (D[p] + D[p+e]) idel^2 (averaged); 3D faces at level 0:
+    D[p] idel^2 (not averaged).  D_xy: 9-point cross terms with coefficient
+    idel_x idel_y / 4 built from D_xy at the face-adjacent nodes."""
+    shape = tuple(m + 3 for m in reversed(ms))
+    idel = [g.idel for g in _grids(ms)]      # x, y, z
+    inner = [np.arange(2, n - 2) for n in shape]
+    idx = -np.ones(shape, dtype=np.int64)
+    mesh = np.meshgrid(*inner, indexing="ij")
+    idx[tuple(mesh)] = np.arange(mesh[0].size).reshape(mesh[0].shape)
+    N = mesh[0].size
+    rows, cols, vals = [], [], []
+    Darr = [np.broadcast_to(np.asarray(d, dtype=float), shape) for d in D]   # D_xx, D_yy[, D_zz]
+    lam_a = np.broadcast_to(np.asarray(lam, dtype=float), shape)
+    for p in zip(*[m.ravel() for m in mesh]):
+        i = idx[p]
+        diag = 0.0
+        for a in range(dim):            # array axis a <-> physical axis dim-1-a
+            phys = dim - 1 - a
+            s = idel[phys] ** 2
+            Dp = Darr[phys]
+            for sgn in (+1, -1):
+                q = list(p)
+                q[a] += sgn
+                q = tuple(q)
+                face_lo = p if sgn > 0 else q
+                face_hi = q if sgn > 0 else p
+                if dim == 2:
+                    f = 0.5 * (Dp[face_lo] + Dp[face_hi]) * s
+                else:
+                    f = Dp[face_lo] * s
+                diag += f
+                if idx[q] >= 0:
+                    rows.append(i); cols.append(idx[q]); vals.append(f)
+        diag += lam_a[p]
+        rows.append(i); cols.append(i); vals.append(-diag)
+        if Dxy is not None:
+            c = idel[0] * idel[1] * 0.25
+            y, x = p
+            Q = Dxy
+            terms = {(-1, -1): (Q[y, x - 1] + Q[y - 1, x]) * c,
+                     (+1, -1): -(Q[y, x - 1] + Q[y + 1, x]) * c,
+                     (-1, +1): -(Q[y, x + 1] + Q[y - 1, x]) * c,
+                     (+1, +1): (Q[y, x + 1] + Q[y + 1, x]) * c}
+            for (dy, dx), v in terms.items():
+                q = (y + dy, x + dx)
+                if idx[q] >= 0:
+                    rows.append(i); cols.append(idx[q]); vals.append(v)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    return A, idx, mesh
+
+
+def _direct_solve(dim, ms, lam, D, rhs, Dxy=None):
+    A, idx, mesh = assemble_level0(dim, ms, lam, D, Dxy)
+    b = rhs[tuple(mesh)].ravel()
+    x = spla.spsolve(A.tocsc(), b)
+    out = np.zeros(rhs.shape)
+    out[tuple(mesh)] = x.reshape(mesh[0].shape)
+    return out, A
+
+
+def _mg_converged(dim, ms, lam, D, rhs, Dxy=None, cycles=200):
+    op = gmg.build_operator(dim, ms, _idels(ms), lam, list(D), Dxy)
+    phi, _ = gmg.solve(op, gmg.Config(fixed=True, max_iters_outer=cycles), np.zeros(rhs.shape), rhs)
+    return phi, op
+
+
+@pytest.mark.parametrize("ms,dxy", [((8, 8), False), ((16, 8), False), ((8, 16), True), ((8, 8), True)])
+def test_2d_brute_force(ms, dxy):
+    rng = np.random.default_rng(sum(ms) + dxy)
+    shape = tuple(m + 3 for m in reversed(ms))
+    D = [np.exp(0.3 * rng.standard_normal(shape)) for _ in range(2)]
+    lam = rng.uniform(0, 2, shape)
+    Dxy = 0.1 * rng.standard_normal(shape) if dxy else None
+    rhs = rng.standard_normal(shape)
+    want, _ = _direct_solve(2, ms, lam, D, rhs, Dxy)
+    got, op = _mg_converged(2, ms, lam, D, rhs, Dxy)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10 * np.abs(want).max())
+    # the oracle's operator application agrees with the assembled matrix
+    A, idx, mesh = assemble_level0(2, ms, lam, D, Dxy)
+    v = gmg.zero_rim(rng.standard_normal(shape))
+    Lv = gmg.apply_operator(op, v)
+    np.testing.assert_allclose(Lv[tuple(mesh)].ravel(), A @ v[tuple(mesh)].ravel(), rtol=1e-13, atol=1e-9)
+
+
+@pytest.mark.parametrize("ms", [(8, 8, 8), (8, 16, 8)])
+def test_3d_brute_force_spatial(ms):
+    rng = np.random.default_rng(7 + sum(ms))
+    shape = tuple(m + 3 for m in reversed(ms))
+    D = [np.exp(0.3 * rng.standard_normal(shape)) for _ in range(3)]
+    lam = rng.uniform(0, 2, shape)
+    rhs = rng.standard_normal(shape)
+    want, A = _direct_solve(3, ms, lam, D, rhs)
+    got, op = _mg_converged(3, ms, lam, D, rhs)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10 * np.abs(want).max())
+    # a 2D-style (averaged) level-0 face would not reproduce this solution
+    Dav = [0.5 * (d + np.roll(d, -1, axis=a)) for a, d in zip((2, 1, 0), D)]
+    wrong, _ = _direct_solve(3, ms, lam, Dav, rhs)
+    assert np.abs(wrong - want).max() > 1e-4 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+def test_3d_constant_D_face_quirk(level):
+    """Constant D arrays: level 0/1 faces equal the scalar faces; at level
+    l >= 2 the face next to the upper boundary is (1/2 + 2^-l) D, because
+    level l is injected from the already averaged level l-1 whose ghost is 0
+    (solver_3d.py:363-370).  Lower faces are unaffected."""
+    ms = (32, 32, 32)
+    c = 1.7
+    shape = (35, 35, 35)
+    D = [np.full(shape, c)] * 3
+    op = gmg.build_operator(3, ms, _idels(ms), 0.0, D)
+    lv = op.lv[level]
+    m = ms[0] >> level
+    sq = (_idels(ms)[0] * 0.5 ** level) ** 2
+    for a in range(3):
+        plus = lv.a_plus[a]          # (m+1)^3, entry k <-> phi index k+1
+        upper = [slice(1, m)] * 3
+        upper[a] = m - 1             # phi index m: face (m, m+1)
+        lower = [slice(1, m)] * 3
+        lower[a] = 1                 # phi index 2: face (2, 3)
+        inner = [slice(1, m)] * 3
+        inner[a] = slice(1, m - 1)
+        expect_upper = (0.5 + 2.0 ** -level) * c * sq if level >= 2 else c * sq
+        np.testing.assert_allclose(plus[tuple(upper)], expect_upper, rtol=1e-15)
+        np.testing.assert_allclose(plus[tuple(lower)], c * sq, rtol=1e-15)
+        np.testing.assert_allclose(plus[tuple(inner)], c * sq, rtol=1e-15)
+
+
+def test_2d_constant_D_arrays_equal_scalar_path():
+    ms = (32, 32)
+    shape = (35, 35)
+    rng = np.random.default_rng(3)
+    rhs = rng.standard_normal(shape)
+    lam = rng.uniform(0, 1, shape)
+    gx, gy = _grids(ms)
+    a, ca = gmg.solve_2d(gx, gy, np.zeros(shape), rhs, lam, np.full(shape, 2.0), np.full(shape, 0.5),
+                         fixed=True, max_iters_outer=8)
+    b, cb = gmg.solve_2d(gx, gy, np.zeros(shape), rhs, lam, 2.0, 0.5, fixed=True, max_iters_outer=8)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * np.abs(b).max())
+
+
+@pytest.mark.parametrize("dim,ms,k", [(2, (32, 32), (1, 2)), (2, (16, 32), (3, 1)), (3, (16, 16, 16), (1, 2, 1))])
+def test_discrete_sine_eigenvector(dim, ms, k):
+    """D scalar, lambda = 0, rhs = -mu s_k with s_k the discrete sine mode:
+    the converged solution is s_k, mu = sum_a 4 D_a idel_a^2 sin^2(k_a pi h_a / 2)."""
+    gr = _grids(ms)
+    Dv = [1.0, 0.7, 1.3][:dim]
+    shape = tuple(m + 3 for m in reversed(ms))
+    s = np.ones(shape)
+    mu = 0.0
+    for phys in range(dim):
+        a = dim - 1 - phys
+        h = gr[phys].del_
+        i = np.arange(shape[a])
+        mode = np.sin(k[phys] * np.pi * (i - 1) * h)
+        sh = [1] * dim
+        sh[a] = shape[a]
+        s = s * mode.reshape(sh)
+        mu += 4 * Dv[phys] * gr[phys].idel ** 2 * np.sin(k[phys] * np.pi * h / 2) ** 2
+    s = gmg.zero_rim(s)
+    rhs = -mu * s
+    op = gmg.build_operator(dim, ms, _idels(ms), 0.0, Dv)
+    phi, _ = gmg.solve(op, gmg.Config(fixed=True, max_iters_outer=60), np.zeros(shape), rhs)
+    np.testing.assert_allclose(phi, s, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("dim,ms", [(2, (8, 8)), (3, (4, 4, 8))])
+def test_operator_symmetric(dim, ms):
+    rng = np.random.default_rng(11)
+    shape = tuple(m + 3 for m in reversed(ms))
+    D = [np.exp(0.5 * rng.standard_normal(shape)) for _ in range(dim)]
+    lam = rng.uniform(0, 1, shape)
+    op = gmg.build_operator(dim, ms, _idels(ms), lam, D)
+    inner = tuple(slice(2, n - 2) for n in shape)
+    n = int(np.prod([n - 4 for n in shape]))
+    M = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(shape)
+        e[inner].flat[j] = 1.0
+        M[:, j] = gmg.apply_operator(op, e)[inner].ravel()
+    np.testing.assert_allclose(M, M.T, rtol=1e-14, atol=0)
+    assert np.all(np.linalg.eigvalsh(M) < 0)    # L is negative definite
+
+
+@pytest.mark.parametrize("dim,ms", [(2, (16, 16)), (3, (8, 8, 8))])
+def test_ghost_values_have_no_effect(dim, ms):
+    rng = np.random.default_rng(5)
+    shape = tuple(m + 3 for m in reversed(ms))
+    D = [np.exp(0.5 * rng.standard_normal(shape)) for _ in range(dim)]
+    lam = rng.uniform(0, 1, shape)
+    rhs = rng.standard_normal(shape)
+    gr = _grids(ms)
+    base = (gmg.solve_2d if dim == 2 else gmg.solve_3d)(*gr, np.zeros(shape), rhs, lam, *D,
+                                                        fixed=True, max_iters_outer=4)[0]
+    ghost = np.zeros(shape, dtype=bool)
+    for a in range(dim):
+        sl = [slice(None)] * dim
+        sl[a] = 0
+        ghost[tuple(sl)] = True
+        sl[a] = -1
+        ghost[tuple(sl)] = True
+
+    def perturb(x):
+        y = x.copy()
+        y[ghost] = rng.uniform(5, 9, ghost.sum())
+        return y
+    got = (gmg.solve_2d if dim == 2 else gmg.solve_3d)(*gr, np.zeros(shape), perturb(rhs), perturb(lam),
+                                                       *[perturb(d) for d in D],
+                                                       fixed=True, max_iters_outer=4)[0]
+    inner = tuple(slice(1, n - 1) for n in shape)
+    np.testing.assert_array_equal(got[inner], base[inner])
