@@ -32,11 +32,63 @@ def parse():
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps per roofline timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cycles", type=int, default=1)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only set up the operator and time the per-level sweeps (for profiling)")
     return ap.parse_args()
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def roofline(a, grids, m):
+    """Per-level RBGS sweep time with HIP events on the stream the kernels run
+    on; the fine level's FP32 sweep is the roofline kernel.  Algorithmic bytes
+    per sweep and unknown: 7 words (phi read + write, rhs, 3 face arrays, the
+    diagonal)."""
+    import torch
+
+    from jsolver_amd import _lib, _solver
+    from jsolver_amd._device import stream_ptr
+    lib = _lib.load()
+    plan = _solver.get_plan(3, (m, m, m), [g.idel for g in grids], "mixed")
+    per_level, roof = [], None
+    for prec_name, prec, w in (("fp32", _lib.JSV_MIXED, 4), ("fp64", _lib.JSV_FP64, 8)):
+        for level in range(plan.levels - 1):
+            mm = m >> level
+            n_unk = (mm - 1) ** 3
+            sweeps = a.sweeps if level < 3 else 50
+            _lib.check(lib.jsv_smooth(plan.h, prec, level, 2, stream_ptr()))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _lib.check(lib.jsv_smooth(plan.h, prec, level, sweeps, stream_ptr()))
+            e1.record()
+            torch.cuda.synchronize()
+            t_sweep = e0.elapsed_time(e1) / 1e3 / sweeps
+            gbs = 7 * w * n_unk / t_sweep / 1e9
+            per_level.append({"prec": prec_name, "level": level, "m": mm,
+                              "us_per_sweep": round(t_sweep * 1e6, 2), "GBps": round(gbs, 1)})
+            if level == 0 and prec_name == "fp32":
+                roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": _traffic(),
+                        "kernel": "k_rb3<float>: one RBGS sweep (red + black launches), fine level",
+                        "algorithmic_bytes_per_sweep": 7 * w * n_unk,
+                        "us_per_sweep": round(t_sweep * 1e6, 2)}
+    log("per-level sweeps: " + ", ".join(f"{p['prec']}/L{p['level']}:{p['GBps']}GB/s" for p in per_level))
+    return roof, per_level
+
+
+def _traffic():
+    """HBM bytes per fine-level FP32 sweep measured with rocprofv3 PMC
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), recorded in
+    profiles/traffic_rb3_fine_256.json; None when absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic_rb3_fine_256.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_sweep"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -45,13 +97,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    import ctypes
-
     import numpy as np
     import torch
 
-    from jsolver_amd import _lib, _solver, fields
-    from jsolver_amd._device import stream_ptr
+    from jsolver_amd import fields
     from jsolver_amd.solver_3d import solve_3d
 
     m = a.m
@@ -67,6 +116,13 @@ def main():
     def run(precision):
         return solve_3d(*grids, phi0, rhs_d, lam_d, dx_d, dy_d, dz_d, precision=precision,
                         inner_cycles=a.inner_cycles, return_info=True, **kw)
+
+    if a.roofline_only:
+        solve_3d(*grids, phi0, rhs_d, lam_d, dx_d, dy_d, dz_d, precision="mixed", fixed=True,
+                 max_iters_outer=1)
+        roof, per_level = roofline(a, grids, m)
+        print(json.dumps({"metric": METRIC, "roofline": roof, "levels": per_level}), flush=True)
+        return
 
     # ---- FP64 reference solve -------------------------------------------
     for _ in range(max(1, a.warmup)):
@@ -91,32 +147,7 @@ def main():
         f"rel res {infom['rel_residual']:.3e}")
     err_vs_fp64 = float(torch.linalg.norm(phim - phi64) / torch.linalg.norm(phi64))
 
-    # ---- roofline: fine-level RBGS sweep, HIP events on the launch stream ----
-    lib = _lib.load()
-    per_level = []
-    roof = None
-    for prec_name, prec, w in (("fp32", _lib.JSV_MIXED, 4), ("fp64", _lib.JSV_FP64, 8)):
-        plan = _solver.get_plan(3, (m, m, m), [g.idel for g in grids], "mixed")
-        for level in range(plan.levels - 1):
-            mm = m >> level
-            n_unk = (mm - 1) ** 3
-            sweeps = a.sweeps if level < 3 else 50
-            _lib.check(lib.jsv_smooth(plan.h, prec, level, 2, stream_ptr()))
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            _lib.check(lib.jsv_smooth(plan.h, prec, level, sweeps, stream_ptr()))
-            e1.record()
-            torch.cuda.synchronize()
-            t_sweep = e0.elapsed_time(e1) / 1e3 / sweeps
-            gbs = 7 * w * n_unk / t_sweep / 1e9
-            per_level.append({"prec": prec_name, "level": level, "m": mm,
-                              "us_per_sweep": round(t_sweep * 1e6, 2), "GBps": round(gbs, 1)})
-            if level == 0 and prec_name == "fp32":
-                roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
-                        "kernel": "k_rb3<float> red+black sweep, fine level",
-                        "algorithmic_bytes_per_launch_pair": 7 * w * n_unk}
-    log("per-level sweeps: " + ", ".join(f"{p['prec']}/L{p['level']}:{p['GBps']}GB/s" for p in per_level))
+    roof, per_level = roofline(a, grids, m)
 
     # ---- CPU baseline: the oracle (NumPy FP64), bounded sample ----------------
     cpu = None

@@ -85,10 +85,31 @@ int jsv_plan_levels(const jsv_plan *plan, int *levels);
 int jsv_plan_level_shape(const jsv_plan *plan, int level, int shape[3]);
 /* bytes of device memory the plan holds */
 int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
-/* Working-set bound of one z-slab in the 3D smoothing schedule (default 96 MiB;
- * 0 = whole level per launch).  Results do not depend on it.  No reference
- * counterpart (a launch-schedule knob). */
-int jsv_plan_set_slab_bytes(jsv_plan *plan, int64_t bytes);
+/* Launch-schedule knobs; results never depend on them.  No reference
+ * counterpart.
+ *   JSV_TUNE_SLAB_BYTES   working set of one z-slab of the 3D two-launch
+ *                         smoothing schedule (default 0 = whole level per launch)
+ *   JSV_TUNE_TWO_PASS_MIN level size (elements) from which the restriction
+ *                         stores first-half residuals and gathers them in a
+ *                         second pass (default 2^20)
+ *   JSV_TUNE_FUSED_MIN_M  3D levels with mz >= this run each RBGS sweep as one
+ *                         fused out-of-place z-marching kernel (default 256;
+ *                         0 = never)
+ *   JSV_TUNE_ZCHUNK       planes per workgroup of that kernel (0 = 16 in FP32, 32 in FP64)
+ *   JSV_TUNE_SWEEP_ROWS   rows per workgroup tile of that kernel: 4, 6, 8, 16
+ *                         (0 = 16 in FP32, 8 in FP64)
+ *   JSV_TUNE_SWEEP_THREADS threads per workgroup of that kernel: 256 or 1024
+ *                         (0 = 1024)
+ *   JSV_TUNE_GRAPHS       1 (default): solves replay each cycle as a captured
+ *                         hipGraph; 0: launch every kernel */
+#define JSV_TUNE_SLAB_BYTES 0
+#define JSV_TUNE_TWO_PASS_MIN 1
+#define JSV_TUNE_FUSED_MIN_M 2
+#define JSV_TUNE_ZCHUNK 3
+#define JSV_TUNE_SWEEP_ROWS 4
+#define JSV_TUNE_SWEEP_THREADS 5
+#define JSV_TUNE_GRAPHS 6
+int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /
  * compute_discretization (solver_2d.py:326-421, solver_3d.py:338-429).

@@ -14,6 +14,13 @@ JSV_FP64 = 0
 JSV_MIXED = 1
 JSV_STOP_DISTANCE = 0
 JSV_STOP_REL_RESIDUAL = 1
+JSV_TUNE_SLAB_BYTES = 0
+JSV_TUNE_TWO_PASS_MIN = 1
+JSV_TUNE_FUSED_MIN_M = 2
+JSV_TUNE_ZCHUNK = 3
+JSV_TUNE_SWEEP_ROWS = 4
+JSV_TUNE_SWEEP_THREADS = 5
+JSV_TUNE_GRAPHS = 6
 JSV_BUF_PHI = 0
 JSV_BUF_RHS = 1
 
@@ -63,7 +70,7 @@ SIGNATURES = {
     "jsv_plan_levels": (_I, [_P, ctypes.POINTER(_I)]),
     "jsv_plan_level_shape": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "jsv_plan_device_bytes": (_I, [_P, ctypes.POINTER(_I64)]),
-    "jsv_plan_set_slab_bytes": (_I, [_P, _I64]),
+    "jsv_plan_set_tuning": (_I, [_P, _I, _I64]),
     "jsv_plan_set_operator": (_I, [_P, _P, _D, _P, _P, _P, _DP, _P, _P]),
     "jsv_solve": (_I, [_P, _P, _P, _P, ctypes.POINTER(JsvConfig), ctypes.POINTER(JsvResult), _P]),
     "jsv_solve_batched": (_I, [_P, _P, _P, _P, ctypes.POINTER(JsvConfig), ctypes.POINTER(JsvResult),

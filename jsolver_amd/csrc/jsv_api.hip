@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/jsolver_amd.h"
@@ -60,6 +61,8 @@ template <typename T>
 struct Hier {
     std::vector<Geo> g;
     std::vector<T*> phi, rhs, cx, cy, cz, cc, drr, dlr;
+    std::vector<T*> res;   // first-half residual scratch of the two-pass restriction (large levels)
+    std::vector<T*> phi2;  // second phi buffer of the fused out-of-place 3D sweep (large levels)
 };
 
 inline dim3 grid_points(const Geo& g, int dim, int xcount)
@@ -102,7 +105,19 @@ struct jsv_plan {
     double* hscal = nullptr;        // pinned host mirror
     int* d_act = nullptr;           // per-problem "still iterating" flags
     int* h_act = nullptr;
-    long slab_bytes = 96L << 20;    // slab size of the 3D smoothing schedule (0: whole level)
+    long slab_bytes = 0;            // slab size of the 3D two-launch smoothing schedule (0: whole level)
+    long two_pass_min = 1L << 20;   // level size from which restriction runs in two passes
+    int fused_min_m = 256;          // 3D levels with mz >= this use the fused out-of-place sweep
+    int zchunk = 0;                 // planes per workgroup of the fused sweep (0: by level size)
+    int sweep_rows = 0;             // rows per workgroup tile of the fused sweep (0: by precision)
+    int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024)
+    bool use_graphs = true;         // replay whole cycles as captured hipGraphs
+    hipStream_t cap_stream = nullptr;   // private stream for graph capture
+    struct Graph {
+        int prec, gamma, fcycle, skip_pre, nu1, nu2, use_act;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
     bool use_act = false;           // kernels consult d_act (dynamic batched solves)
     std::vector<void*> allocs;
     int64_t bytes = 0;
@@ -156,12 +171,57 @@ int alloc_hier(jsv_plan* P, Hier<T>& H, bool with_coeffs)
     RETCHK(one(H.cc, with_coeffs));
     RETCHK(one(H.drr, false));
     RETCHK(one(H.dlr, false));
+    // two-pass restriction on levels above ~1M points (one-pass below: there
+    // the stencil re-reads hit in L2 and one launch is cheaper)
+    H.res.assign(L, nullptr);
+    for (int l = 0; l + 1 < L; ++l)
+        if (H.g[l].n >= P->two_pass_min)
+            RETCHK(dalloc(P, (void**)&H.res[l], sizeof(T) * H.g[l].n));
+    H.phi2.assign(L, nullptr);
+    if (P->dim == 3)
+        for (int l = 0; l < L; ++l)
+            if (P->fused_min_m > 0 && H.g[l].mz >= P->fused_min_m)
+                RETCHK(dalloc(P, (void**)&H.phi2[l], sizeof(T) * H.g[l].n));
     return 0;
 }
 
 // ---------------------------------------------------------------------------
 // elementary operations
 // ---------------------------------------------------------------------------
+template <typename T, int TY, int NT>
+int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+{
+    const Geo& g = H.g[l];
+    const int ytiles = (g.my - 1 + TY - 1) / TY;
+    int zchunk = P->zchunk > 0 ? P->zchunk : (sizeof(T) == 4 ? 16 : 32);
+    const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
+    const int nh = (g.nx + 1) / 2;
+    const size_t lds = sizeof(T) * (size_t)(3 * (TY + 4) + 3 * (TY + 2)) * nh;
+    if (lds > 160 * 1024) return fail("fused sweep tile needs %zu bytes of LDS", lds);
+    hipLaunchKernelGGL((jsv::k_rbsweep3<T, TY, NT>), dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds,
+                       s, H.phi[l], H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk,
+                       ytiles);
+    return 0;
+}
+
+// tile rows x threads per workgroup of the fused sweep (tunable; default by precision)
+template <typename T>
+int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+{
+    int rows = P->sweep_rows > 0 ? P->sweep_rows : (sizeof(T) == 4 ? 16 : 8);
+    const int nt = P->sweep_threads > 0 ? P->sweep_threads : 1024;
+    if (nt == 256) {
+        if (rows == 4) return launch_rbsweep3_t<T, 4, 256>(P, H, l, s);
+        if (rows == 6) return launch_rbsweep3_t<T, 6, 256>(P, H, l, s);
+        if (rows == 8) return launch_rbsweep3_t<T, 8, 256>(P, H, l, s);
+        return launch_rbsweep3_t<T, 16, 256>(P, H, l, s);
+    }
+    if (rows == 4) return launch_rbsweep3_t<T, 4, 1024>(P, H, l, s);
+    if (rows == 6) return launch_rbsweep3_t<T, 6, 1024>(P, H, l, s);
+    if (rows == 8) return launch_rbsweep3_t<T, 8, 1024>(P, H, l, s);
+    return launch_rbsweep3_t<T, 16, 1024>(P, H, l, s);
+}
+
 template <typename T>
 int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
 {
@@ -186,6 +246,10 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
             hipLaunchKernelGGL(jsv::k_rb2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
                                H.rhs[l], H.cx[l], H.cy[l], H.cc[l], g, 1, P->act_ptr());
         }
+    } else if (P->fused_min_m > 0 && g.mz >= P->fused_min_m && H.phi2[l] != nullptr) {
+        // fused out-of-place sweep, then swap the level's phi buffers
+        RETCHK(launch_rbsweep3<T>(P, H, l, s));
+        std::swap(H.phi[l], H.phi2[l]);
     } else {
         // Slab schedule: red on z-slab k, then black on slab k-1, so the black
         // half re-reads a slab the red half fetched two launches earlier (still
@@ -226,6 +290,24 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
     const Geo& gf = H.g[l];
     const Geo& gc = H.g[l + 1];
     dim3 gr = grid_points(gc, P->dim, gc.mx - 1);
+    if (H.res[l] != nullptr) {
+        // two passes: residual at the first-half points, then the weighting
+        dim3 gf_half = grid_points(gf, P->dim, gf.mx / 2);
+        if (P->dim == 2) {
+            hipLaunchKernelGGL(jsv::k_resid_first2<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
+                               H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cc[l],
+                               P->offdiag ? H.drr[l] : nullptr, P->offdiag ? H.dlr[l] : nullptr, gf,
+                               P->act_ptr());
+            hipLaunchKernelGGL(jsv::k_restrict_gather2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
+                               H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr());
+        } else {
+            hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
+                               H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf);
+            hipLaunchKernelGGL(jsv::k_restrict_gather3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
+                               H.rhs[l + 1], H.res[l], gf, gc);
+        }
+        return 0;
+    }
     if (P->dim == 2)
         hipLaunchKernelGGL(jsv::k_restrict2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
                            H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cc[l],
@@ -280,6 +362,69 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
     RETCHK(prolong(P, H, l, c.nu2, s));
     for (int i = 0; i < c.nu2; ++i) RETCHK(smooth(P, H, l, s));
     return 0;
+}
+
+void drop_graphs(jsv_plan* P)
+{
+    for (auto& gr : P->graphs) (void)hipGraphExecDestroy(gr.exec);
+    P->graphs.clear();
+}
+
+// One top-level cycle.  The kernel sequence of a cycle depends only on the
+// configuration, so it is captured once into a hipGraph (on the plan's private
+// stream; capture on the legacy default stream is not allowed) and replayed on
+// the caller's stream: one submission instead of ~50 launches, which keeps the
+// coarse levels' tiny kernels from starving the GPU.  The cycle's kernels
+// record no memset or memcpy nodes.  A configuration whose sweeps leave a
+// level's phi in its second buffer (odd sweep count per visit) is not graph
+// replayable and runs eagerly.
+template <typename T>
+int run_cycle(jsv_plan* P, Hier<T>& H, const jsv_config& c, bool skip_pre, hipStream_t s)
+{
+    const int prec = sizeof(T) == 4 ? JSV_MIXED : JSV_FP64;
+    if (!P->use_graphs) return mg_cycle(P, H, c, 0, c.fcycle != 0, skip_pre, s);
+    const int ua = P->use_act ? 1 : 0;
+    for (auto& gr : P->graphs)
+        if (gr.prec == prec && gr.gamma == c.gamma && gr.fcycle == (c.fcycle != 0) &&
+            gr.skip_pre == (skip_pre ? 1 : 0) && gr.nu1 == c.nu1 && gr.nu2 == c.nu2 && gr.use_act == ua) {
+            if (gr.exec == nullptr) return mg_cycle(P, H, c, 0, c.fcycle != 0, skip_pre, s);
+            HIPCHK(hipGraphLaunch(gr.exec, s));
+            return 0;
+        }
+    if (P->cap_stream == nullptr) HIPCHK(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
+    // the capture stream must not run ahead of work already queued on s
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, s));
+    HIPCHK(hipStreamWaitEvent(P->cap_stream, ev, 0));
+    (void)hipEventDestroy(ev);
+    std::vector<T*> before = H.phi;
+    HIPCHK(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = mg_cycle(P, H, c, 0, c.fcycle != 0, skip_pre, P->cap_stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(P->cap_stream, &graph);
+    if (rc != 0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        H.phi = before;
+        return rc;
+    }
+    if (e != hipSuccess) return fail("graph capture failed: %s", hipGetErrorString(e));
+    jsv_plan::Graph gr{prec, c.gamma, c.fcycle != 0 ? 1 : 0, skip_pre ? 1 : 0, c.nu1, c.nu2, ua, nullptr};
+    const bool same = (H.phi == before);
+    if (same) {
+        e = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail("graph instantiate failed: %s", hipGetErrorString(e));
+    } else {
+        (void)hipGraphDestroy(graph);
+        H.phi = before;                          // replay eagerly instead
+    }
+    P->graphs.push_back(gr);
+    if (gr.exec != nullptr) {
+        HIPCHK(hipGraphLaunch(gr.exec, s));
+        return 0;
+    }
+    return mg_cycle(P, H, c, 0, c.fcycle != 0, skip_pre, s);
 }
 
 int zero_rim_level0(jsv_plan* P, double* a, hipStream_t s)
@@ -448,6 +593,8 @@ int jsv_plan_create(int dim, int mx, int my, int mz, double idel_x, double idel_
 int jsv_plan_destroy(jsv_plan* P)
 {
     if (P == nullptr) return 0;
+    for (auto& gr : P->graphs) (void)hipGraphExecDestroy(gr.exec);
+    if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
     for (void* p : P->allocs) (void)hipFree(p);
     if (P->hscal) (void)hipHostFree(P->hscal);
     if (P->h_act) (void)hipHostFree(P->h_act);
@@ -473,12 +620,67 @@ int jsv_plan_level_shape(const jsv_plan* P, int level, int shape[3])
     return 0;
 }
 
-int jsv_plan_set_slab_bytes(jsv_plan* P, int64_t bytes)
+int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
 {
     if (P == nullptr) return fail("plan is NULL");
-    if (bytes < 0) return fail("slab bytes must be >= 0");
-    P->slab_bytes = (long)bytes;
-    return 0;
+    if (value < 0) return fail("tuning values must be >= 0");
+    drop_graphs(P);    // captured launch shapes depend on every knob
+    if (key == JSV_TUNE_GRAPHS) {
+        P->use_graphs = value != 0;
+        return 0;
+    }
+    if (key == JSV_TUNE_SLAB_BYTES) {
+        P->slab_bytes = (long)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_TWO_PASS_MIN) {
+        P->two_pass_min = (long)value;
+        auto realloc_res = [&](auto& H) -> int {
+            for (int l = 0; l + 1 < P->levels && l < (int)H.res.size(); ++l)
+                if (H.res[l] == nullptr && H.g[l].n >= P->two_pass_min)
+                    RETCHK(dalloc(P, (void**)&H.res[l], sizeof(*H.phi[0]) * H.g[l].n));
+            return 0;
+        };
+        RETCHK(realloc_res(P->h64));
+        if (P->prec == JSV_MIXED) RETCHK(realloc_res(P->h32));
+        // levels below the threshold go back to one pass
+        for (int l = 0; l + 1 < P->levels; ++l) {
+            if (P->h64.g[l].n < P->two_pass_min) {
+                if (l < (int)P->h64.res.size()) P->h64.res[l] = nullptr;
+                if (l < (int)P->h32.res.size()) P->h32.res[l] = nullptr;
+            }
+        }
+        return 0;
+    }
+    if (key == JSV_TUNE_FUSED_MIN_M) {
+        P->fused_min_m = (int)value;
+        auto realloc_phi2 = [&](auto& H) -> int {
+            if (P->dim != 3) return 0;
+            for (int l = 0; l < P->levels && l < (int)H.phi2.size(); ++l)
+                if (H.phi2[l] == nullptr && value > 0 && H.g[l].mz >= value)
+                    RETCHK(dalloc(P, (void**)&H.phi2[l], sizeof(*H.phi[0]) * H.g[l].n));
+            return 0;
+        };
+        RETCHK(realloc_phi2(P->h64));
+        if (P->prec == JSV_MIXED) RETCHK(realloc_phi2(P->h32));
+        return 0;
+    }
+    if (key == JSV_TUNE_ZCHUNK) {
+        P->zchunk = (int)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_SWEEP_ROWS) {
+        if (value != 0 && value != 4 && value != 6 && value != 8 && value != 16)
+            return fail("sweep rows must be 0, 4, 6, 8 or 16");
+        P->sweep_rows = (int)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_SWEEP_THREADS) {
+        if (value != 0 && value != 256 && value != 1024) return fail("sweep threads must be 0, 256 or 1024");
+        P->sweep_threads = (int)value;
+        return 0;
+    }
+    return fail("unknown tuning key %d", key);
 }
 
 int jsv_plan_device_bytes(const jsv_plan* P, int64_t* bytes)
@@ -503,6 +705,7 @@ int jsv_plan_set_operator(jsv_plan* P, const double* lam, double lam_scalar, con
     if (!spatial && d_scalar == nullptr) return fail("scalar D values missing");
     if (dxy != nullptr && (dim != 2 || !spatial)) return fail("D_xy needs 2D array diffusion");
     P->spatial = spatial;
+    if (P->offdiag != (dxy != nullptr)) drop_graphs(P);   // the smoother's kernels change
     P->offdiag = dxy != nullptr;
     Hier<double>& H = P->h64;
     const int L = P->levels;
@@ -851,8 +1054,7 @@ static int solve_fp64(jsv_plan* P, const double* phi_in, const double* rhs, doub
     }
     int rc = 0;
     if (c->fixed) {
-        for (int64_t k = 0; k < c->max_iters && rc == 0; ++k) rc = mg_cycle(P, H, *c, 0, c->fcycle != 0,
-                                                                           c->skip_pre != 0, s);
+        for (int64_t k = 0; k < c->max_iters && rc == 0; ++k) rc = run_cycle(P, H, *c, c->skip_pre != 0, s);
         for (int b = 0; b < B; ++b) T.count[b] = T.outer[b] = c->max_iters;
     } else {
         const double thr = relres ? c->rtol : c->epsilon;
@@ -864,7 +1066,7 @@ static int solve_fp64(jsv_plan* P, const double* phi_in, const double* rhs, doub
             if (!relres && it > 0)
                 HIPCHK(hipMemcpyAsync(P->phi_old, H.phi[0], sizeof(double) * n,
                                       hipMemcpyDeviceToDevice, s));
-            rc = mg_cycle(P, H, *c, 0, c->fcycle != 0, c->skip_pre != 0, s);
+            rc = run_cycle(P, H, *c, c->skip_pre != 0, s);
             if (rc) break;
             LAUNCHCHK();
             if (relres) RETCHK(residual64<double>(P, H.phi[0], H.rhs[0], (double*)nullptr, 0, s));
@@ -957,7 +1159,7 @@ static int solve_mixed(jsv_plan* P, const double* phi_in, const double* rhs, dou
         int64_t todo = k_inner;
         for (int b = 0; b < B; ++b)
             if (P->h_act[b] && T.count[b] + todo > c->max_iters) todo = c->max_iters - T.count[b];
-        for (int64_t k = 0; k < todo && rc == 0; ++k) rc = mg_cycle(P, F, *c, 0, c->fcycle != 0, false, s);
+        for (int64_t k = 0; k < todo && rc == 0; ++k) rc = run_cycle(P, F, *c, false, s);
         if (rc) break;
         if (!relres)
             HIPCHK(hipMemcpyAsync(P->phi_old, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));

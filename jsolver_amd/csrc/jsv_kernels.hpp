@@ -117,6 +117,154 @@ k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
 }
 
 // ---------------------------------------------------------------------------
+// Fused RBGS sweep (red half then black half), out of place, marching in z.
+//
+// One workgroup owns TY rows of the level (all of x) and a chunk of planes
+// [z0, z1).  Per step it loads the black values of phi_in on plane z+1 (rows
+// y0-2 .. y0+TY+1) into an LDS ring, computes the new red values of plane z on
+// rows y0-1 .. y0+TY (one halo row each side, recomputed by the neighbour
+// tile too, from the same inputs, so both get identical bits) into a second
+// LDS ring, then the new black values of plane z-1 on its own rows from the
+// red ring, and writes both colours of plane z-1 to phi_out.  phi_in is only
+// read, phi_out only written, so tiles never race; each array crosses HBM
+// about once per sweep instead of once per half.  Both rings store one colour
+// compactly: the point (x, y, z) of that colour lives at column x >> 1.
+// "Red" is the first half of the reference's sub-colour order (x+y+z odd).
+// Rim values of phi_out are never written (zero from allocation).
+// ---------------------------------------------------------------------------
+extern __shared__ double jsv_dyn_smem[];
+
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
+// remap the launch index so XCD k works on a contiguous run of tiles and the
+// halo rows neighbouring tiles share are fetched once per L2.
+__device__ __forceinline__ int xcd_tile(int id, int n)
+{
+    const int per = n >> 3;
+    return id < (per << 3) ? (id & 7) * per + (id >> 3) : id;
+}
+
+template <typename T, int TY, int NT>
+__global__ void __launch_bounds__(NT)
+k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
+           const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+           const T* __restrict__ cc, Geo g, int zchunk, int ytiles)
+{
+    constexpr int PR = TY + 4;                  // rows of the phi_in ring: y0-2 .. y0+TY+1
+    constexpr int RR = TY + 2;                  // rows of the red ring:    y0-1 .. y0+TY
+    constexpr int NHMAX = 258;                  // compact columns covered by the register prefetch (m <= 512)
+    constexpr int PMAX = (PR * NHMAX + NT - 1) / NT;
+    const int nh = (g.nx + 1) >> 1;             // compact columns per row
+    T* P = reinterpret_cast<T*>(jsv_dyn_smem);  // [3][PR][nh]  black values of phi_in
+    T* R = P + 3 * PR * nh;                     // [3][RR][nh]  new red values
+    const int tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int y0 = 2 + (tile % ytiles) * TY;
+    const int z0 = 2 + (tile / ytiles) * zchunk;
+    int z1 = z0 + zchunk;
+    if (z1 > g.mz + 1) z1 = g.mz + 1;
+    const long sy = g.sy, sz = g.sz;
+    const bool prefetch = nh <= NHMAX;
+
+    for (int i = tid; i < 3 * RR * nh; i += NT) R[i] = (T)0;
+
+    // black values (x+y+z even) of phi_in on plane z, element i of the slot
+    auto fetch = [&](int z, int i) -> T {
+        const int r = i / nh, k = i - r * nh;
+        const int y = y0 - 2 + r;
+        const int x = 2 * k + ((y + z) & 1);
+        if (z >= 0 && z < g.nz && y >= 0 && y < g.ny && x < g.nx) return pin[(long)z * sz + (long)y * sy + x];
+        return (T)0;
+    };
+    auto load_plane = [&](int z) {
+        T* dst = P + (z % 3) * PR * nh;
+        for (int i = tid; i < PR * nh; i += NT) dst[i] = fetch(z, i);
+    };
+    // next plane held in registers while the current one is computed
+    T pre[PMAX];
+    auto issue = [&](int z) {
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            const int i = tid + j * NT;
+            pre[j] = (i < PR * nh) ? fetch(z, i) : (T)0;
+        }
+    };
+    auto commit = [&](int z) {
+        T* dst = P + (z % 3) * PR * nh;
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            const int i = tid + j * NT;
+            if (i < PR * nh) dst[i] = pre[j];
+        }
+    };
+    auto Pv = [&](int z, int y, int x) -> T {     // black phi_in at (x, y, z)
+        return P[((z % 3) * PR + (y - y0 + 2)) * nh + (x >> 1)];
+    };
+    auto Rv = [&](int z, int y, int x) -> T {     // new red at (x, y, z)
+        return R[((z % 3) * RR + (y - y0 + 1)) * nh + (x >> 1)];
+    };
+
+    load_plane(z0 - 2);
+    load_plane(z0 - 1);
+    if (prefetch) issue(z0);
+    for (int zs = z0 - 1; zs <= z1; ++zs) {
+        if (prefetch) {
+            commit(zs + 1);
+            if (zs + 2 <= z1 + 1) issue(zs + 2);      // in flight during this step's compute
+        } else {
+            load_plane(zs + 1);
+        }
+        __syncthreads();
+        // ---- red on plane zs, rows y0-1 .. y0+TY
+        {
+            T* Rs = R + (zs % 3) * RR * nh;
+            const bool zok = (zs >= 2 && zs <= g.mz);
+#pragma unroll 4
+            for (int i = tid; i < RR * nh; i += NT) {
+                const int r = i / nh, k = i - r * nh;
+                const int y = y0 - 1 + r;
+                const int x = 2 * k + ((1 + y + zs) & 1);   // x + y + z odd
+                if (!zok || y < 2 || y > g.my || x < 2 || x > g.mx) {
+                    Rs[i] = (T)0;
+                    continue;
+                }
+                const long p = (long)zs * sz + (long)y * sy + x;
+                T acc = cz[p] * Pv(zs + 1, y, x) + cz[p - sz] * Pv(zs - 1, y, x);
+                acc = acc + cy[p] * Pv(zs, y + 1, x);
+                acc = acc + cy[p - sy] * Pv(zs, y - 1, x);
+                acc = acc + cx[p] * Pv(zs, y, x + 1);
+                acc = acc + cx[p - 1] * Pv(zs, y, x - 1);
+                acc = acc - rhs[p];
+                Rs[i] = acc / cc[p];
+            }
+        }
+        __syncthreads();
+        // ---- black on plane zb = zs-1, rows y0 .. y0+TY-1; write both colours
+        const int zb = zs - 1;
+        if (zb >= z0 && zb < z1) {
+#pragma unroll 4
+            for (int i = tid; i < TY * g.nx; i += NT) {
+                const int r = i / g.nx, x = i - r * g.nx;
+                const int y = y0 + r;
+                if (y > g.my || x < 2 || x > g.mx) continue;
+                const long q = (long)zb * sz + (long)y * sy + x;
+                if (((x + y + zb) & 1) == 1) {          // red: already computed
+                    pout[q] = Rv(zb, y, x);
+                    continue;
+                }
+                T acc = cz[q] * Rv(zb + 1, y, x) + cz[q - sz] * Rv(zb - 1, y, x);
+                acc = acc + cy[q] * Rv(zb, y + 1, x);
+                acc = acc + cy[q - sy] * Rv(zb, y - 1, x);
+                acc = acc + cx[q] * Rv(zb, y, x + 1);
+                acc = acc + cx[q - 1] * Rv(zb, y, x - 1);
+                acc = acc - rhs[q];
+                pout[q] = acc / cc[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Residual rhs - L phi at one point (solver_2d.py:637, solver_3d.py:642), in the
 // reference's term order.
 // ---------------------------------------------------------------------------
@@ -214,6 +362,84 @@ k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__
     const T rcen = JSV_R(0);
 #undef JSV_R
     rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * rcen;
+}
+
+// Two-pass restriction for large levels: pass 1 stores the residual at every
+// point of the first RBGS half (exactly the points the partial full-weighting
+// reads) once; pass 2 combines them per coarse unknown in the reference's
+// order.  Same bits as the one-pass kernels above, without evaluating each
+// edge residual four times.
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
+               const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+               const T* __restrict__ cc, Geo g)
+{
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int z = 2 + blockIdx.z;
+    if (y > g.my) return;
+    const int x = 2 + ((1 + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (x > g.mx) return;
+    const long p = (long)z * g.sz + (long)y * g.sy + x;
+    r[p] = resid3<T, T>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc)
+{
+    const int I = 2 + blockIdx.z;
+    const int J = 2 + blockIdx.y * BY + threadIdx.y;
+    const int K = 2 + blockIdx.x * BX + threadIdx.x;
+    if (J > gc.my || K > gc.mx) return;
+    const long sy = gf.sy, sz = gf.sz;
+    const long c = (long)(2 * I - 1) * sz + (long)(2 * J - 1) * sy + (2 * K - 1);
+    T e = r[c + sy + 1] + r[c + sy - 1];
+    e = e + r[c - sy + 1];
+    e = e + r[c - sy - 1];
+    e = e + r[c + sz + 1];
+    e = e + r[c + sz - 1];
+    e = e + r[c - sz + 1];
+    e = e + r[c - sz - 1];
+    e = e + r[c + sz + sy];
+    e = e + r[c + sz - sy];
+    e = e + r[c - sz + sy];
+    e = e + r[c - sz - sy];
+    rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * r[c];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_resid_first2(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
+               const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cc,
+               const T* __restrict__ drr, const T* __restrict__ dlr, Geo g, const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    if (y > g.my) return;
+    const int x = 2 + (y & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (x > g.mx) return;
+    const long p = (long)b * g.sz + (long)y * g.sy + x;
+    r[p] = resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, p, g.sy);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_restrict_gather2(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc,
+                   const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int J = 2 + blockIdx.y * BY + threadIdx.y;
+    const int K = 2 + blockIdx.x * BX + threadIdx.x;
+    if (J > gc.my || K > gc.mx) return;
+    const long sy = gf.sy;
+    const long c = (long)b * gf.sz + (long)(2 * J - 1) * sy + (2 * K - 1);
+    T s = r[c + sy + 1] + r[c + sy - 1];
+    s = s + r[c - sy + 1];
+    s = s + r[c - sy - 1];
+    rc[(long)b * gc.sz + (long)J * gc.sy + K] = (T)0.0625 * s + (T)0.25 * r[c];
 }
 
 // ---------------------------------------------------------------------------

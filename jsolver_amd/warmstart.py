@@ -34,7 +34,7 @@ LAYERS = [
     ("d3a", 128, 64, 3, 1), ("d3b", 64, 64, 3, 1),
     ("d2a", 96, 32, 3, 1), ("d2b", 32, 32, 3, 1),
     ("d1a", 48, 16, 3, 1), ("d1b", 16, 16, 3, 1),
-    ("out", 16, 1, 1, 0),
+    ("out", 16, 4, 1, 0),    # channel 0 is phi; 1-3 stay zero (keeps Cin of its input gradient a multiple of 4)
 ]
 
 
@@ -88,6 +88,8 @@ class Layer:
         std = math.sqrt(2.0 / fan_in) if relu else math.sqrt(1.0 / fan_in)
         w = torch.zeros(cout, self.kp, dtype=torch.float32)
         w[:, :fan_in] = torch.randn(cout, fan_in, generator=gen) * std
+        if name == "out":
+            w[1:] = 0.0
         self.w = w.to(dev)                          # fp32 master, layout [co][tap*cin+ci] (padded)
         self.b = torch.zeros(cout, dtype=torch.float32, device=dev)
         self.w16 = self.w.to(BF16)
@@ -206,7 +208,7 @@ class UNet:
         _, out = self._conv(Ls["out"], a["d1b"], N, H, W, out32=True)
         if keep:
             self._acts = a
-        return out.reshape(N, H, W)
+        return out[..., 0].contiguous()
 
     def backward(self, dout):
         """dout: fp32 (N, H, W) gradient of the loss w.r.t. the output.
@@ -215,7 +217,8 @@ class UNet:
         N, H, W, _ = a["x"].shape
         H2, W2, H3, W3, H4, W4 = H // 2, W // 2, H // 4, W // 4, H // 8, W // 8
         Ls = self.layers
-        d = dout.reshape(N, H, W, 1).to(BF16).contiguous()
+        d = torch.zeros((N, H, W, 4), dtype=BF16, device=self.dev)
+        d[..., 0] = dout.to(BF16)
         d = self._conv_bwd(Ls["out"], a["d1b"], None, d, N, H, W)
         d = self._conv_bwd(Ls["d1b"], a["d1a"], a["d1b"], d, N, H, W)
         d = self._conv_bwd(Ls["d1a"], a["u1"], a["d1a"], d, N, H, W)

@@ -71,7 +71,7 @@ def test_smooth_3d_slab_schedule_bitexact(slab_bytes):
     from jsolver_amd import _lib
     case = ("3d_spatial_16x32x32", 3, (16, 32, 32), True, True, False)
     dp, op, _, _ = _setup(case)
-    _lib.check(_lib.load().jsv_plan_set_slab_bytes(dp.plan.h, slab_bytes))
+    _lib.check(_lib.load().jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SLAB_BYTES, slab_bytes))
     for level in range(2):
         phi, rhs = _level_arrays(op, level, 60 + level)
         dp.upload(level, 0, phi)
@@ -94,6 +94,59 @@ def test_restrict_bitexact(case):
         got = dp.download(level + 1, 1)
         want = gmg.restrict(op, level, phi, rhs)
         np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
+
+
+@pytest.mark.parametrize("zchunk", [1, 2, 3, 5, 0])
+@pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 8), (8, 8, 8)])
+def test_fused_sweep_bitexact(ms, zchunk):
+    """The fused out-of-place z-marching sweep (forced on every level, several
+    y-tiles and z-chunk sizes) gives the oracle's bits for sweeps and cycles."""
+    from jsolver_amd import _lib
+    case = ("3d_fused", 3, ms, True, True, False)
+    dp, op, phi0, rhs0 = _setup(case, seed=13)
+    lib = _lib.load()
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSED_MIN_M, 2))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_ZCHUNK, zchunk))
+    for level in range(op.levels - 1):
+        phi, rhs = _level_arrays(op, level, 80 + level)
+        dp.upload(level, 0, phi)
+        dp.upload(level, 1, rhs)
+        dp.smooth(level, 3)
+        want = phi
+        for _ in range(3):
+            want = gmg.smooth(op, level, want, rhs)
+        np.testing.assert_array_equal(dp.download(level, 0), want, err_msg=f"level {level}")
+    for kw in (dict(), dict(gamma=2, fcycle=True), dict(nu1=2, nu2=0)):
+        dp.upload(0, 0, phi0)
+        dp.upload(0, 1, rhs0)
+        dp.cycle(cfg(**kw))
+        c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+        np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_restrict_two_pass_bitexact(case):
+    """The two-pass restriction of large levels (residuals of the first RBGS
+    half stored, then weighted) gives the oracle's bits; forced on every level."""
+    from jsolver_amd import _lib
+    dp, op, _, _ = _setup(case)
+    _lib.check(_lib.load().jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_TWO_PASS_MIN, 1))
+    for level in range(op.levels - 2):
+        phi, rhs = _level_arrays(op, level, 20 + level)
+        dp.upload(level, 0, phi)
+        dp.upload(level, 1, rhs)
+        dp.upload(level + 1, 1, np.zeros(op.lv[level + 1].shape))
+        dp.restrict(level)
+        got = dp.download(level + 1, 1)
+        want = gmg.restrict(op, level, phi, rhs)
+        np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
+    # and whole cycles with it
+    phi, rhs = _level_arrays(op, 0, 70)
+    dp.upload(0, 0, phi)
+    dp.upload(0, 1, rhs)
+    dp.cycle(cfg(gamma=2))
+    c = gmg.Config(fixed=True, max_iters_outer=1, gamma=2)
+    np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, phi, rhs))
 
 
 @pytest.mark.parametrize("nu2", [1, 0])
