@@ -28,7 +28,8 @@ def parse():
     ap.add_argument("--m", type=int, default=256)
     ap.add_argument("--cycle", choices=["V", "F"], default="V")
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--inner-cycles", type=int, default=4)
+    ap.add_argument("--inner-cycles", type=int, default=0,
+                    help="FP32 cycles per refinement step (0: predicted from the contraction)")
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps per roofline timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cycles", type=int, default=1)
@@ -113,9 +114,9 @@ def main():
     unknowns = (m - 1) ** 3
     kw = dict(stop="rel_residual", rtol=a.rtol, fcycle=(a.cycle == "F"))
 
-    def run(precision):
+    def run(precision, check_every=1):
         return solve_3d(*grids, phi0, rhs_d, lam_d, dx_d, dy_d, dz_d, precision=precision,
-                        inner_cycles=a.inner_cycles, return_info=True, **kw)
+                        inner_cycles=a.inner_cycles, check_every=check_every, return_info=True, **kw)
 
     if a.roofline_only:
         solve_3d(*grids, phi0, rhs_d, lam_d, dx_d, dy_d, dz_d, precision="mixed", fixed=True,
@@ -133,6 +134,14 @@ def main():
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t0
     log(f"fp64: {n64} cycles, {t64 * 1e3:.1f} ms, rel res {info64['rel_residual']:.3e}")
+    # FP64 with the residual checks spaced like the mixed solve's refinement steps
+    run("fp64", check_every=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, n64a, info64a = run("fp64", check_every=0)
+    torch.cuda.synchronize()
+    t64a = time.perf_counter() - t0
+    log(f"fp64 (adaptive checks): {n64a} cycles, {t64a * 1e3:.1f} ms")
 
     # ---- mixed precision: the timed steps ----------------------------------
     for _ in range(a.warmup):
@@ -190,7 +199,10 @@ def main():
                                f"dynamic to rel. residual {a.rtol:g}",
                    "unknowns": unknowns, "cycle": a.cycle, "inner_cycles": a.inner_cycles},
         "fp64": {"time_to_solution_s": round(t64, 5), "cycles": n64,
-                 "rel_residual": info64["rel_residual"]},
+                 "rel_residual": info64["rel_residual"], "residual_checked": "every cycle"},
+        "fp64_adaptive_checks": {"time_to_solution_s": round(t64a, 5), "cycles": n64a,
+                                 "rel_residual": info64a["rel_residual"],
+                                 "speedup_mixed_vs_this": round(t64a / tm, 3)},
         "mixed": {"time_to_solution_s": round(tm, 5), "fp32_cycles": nm,
                   "refinements": infom["outer_steps"], "rel_residual": infom["rel_residual"],
                   "rel_error_vs_fp64": err_vs_fp64},

@@ -49,8 +49,11 @@ typedef struct jsv_config {
     int64_t max_iters;      /* max outer cycles (int32 max = unbounded) */
     int stop;               /* JSV_STOP_* (dynamic mode only) */
     double rtol;            /* relative-residual threshold for JSV_STOP_REL_RESIDUAL */
-    int inner_cycles;       /* JSV_MIXED: FP32 cycles per FP64 refinement step */
-    int check_every;        /* dynamic mode: cycles launched between host checks (>=1) */
+    int inner_cycles;       /* JSV_MIXED: FP32 cycles per FP64 refinement step;
+                               0 = predicted from the measured contraction (8..64) */
+    int check_every;        /* JSV_FP64 with JSV_STOP_REL_RESIDUAL: cycles between
+                               residual checks (1 = every cycle, the exact count);
+                               0 = spaced by the predicted cycles-to-tolerance */
     double *history;        /* optional host array: per-cycle distance or relative residual */
     int64_t history_len;    /* capacity of history */
 } jsv_config;
@@ -97,11 +100,15 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         0 = never)
  *   JSV_TUNE_ZCHUNK       planes per workgroup of that kernel (0 = 16 in FP32, 32 in FP64)
  *   JSV_TUNE_SWEEP_ROWS   rows per workgroup tile of that kernel: 4, 6, 8, 16
- *                         (0 = 16 in FP32, 8 in FP64)
+ *                         (0 = 16 in FP32, 4 in FP64)
  *   JSV_TUNE_SWEEP_THREADS threads per workgroup of that kernel: 256 or 1024
  *                         (0 = 1024)
  *   JSV_TUNE_GRAPHS       1 (default): solves replay each cycle as a captured
- *                         hipGraph; 0: launch every kernel */
+ *                         hipGraph; 0: launch every kernel
+ *   JSV_TUNE_COARSE_M     levels whose largest resolution is <= this (and
+ *                         every coarser level) run each coarse sub-cycle as
+ *                         one single-workgroup launch per problem
+ *                         (default 16 in 3D, 64 in 2D; 0 = off) */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -109,6 +116,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_SWEEP_ROWS 4
 #define JSV_TUNE_SWEEP_THREADS 5
 #define JSV_TUNE_GRAPHS 6
+#define JSV_TUNE_COARSE_M 7
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

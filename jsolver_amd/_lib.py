@@ -21,6 +21,7 @@ JSV_TUNE_ZCHUNK = 3
 JSV_TUNE_SWEEP_ROWS = 4
 JSV_TUNE_SWEEP_THREADS = 5
 JSV_TUNE_GRAPHS = 6
+JSV_TUNE_COARSE_M = 7
 JSV_BUF_PHI = 0
 JSV_BUF_RHS = 1
 

@@ -132,7 +132,7 @@ def clear_plans():
 # the solve (Solver_2D.solve / Solver_3D.solve)
 # ---------------------------------------------------------------------------
 def make_config(fixed, epsilon, gamma, fcycle, skip_pre, nu1, nu2, max_iters_outer, stop, rtol,
-                inner_cycles, history=None):
+                inner_cycles, history=None, check_every=1):
     c = _lib.JsvConfig()
     c.fixed = 1 if fixed else 0
     c.epsilon = float(epsilon)
@@ -145,7 +145,7 @@ def make_config(fixed, epsilon, gamma, fcycle, skip_pre, nu1, nu2, max_iters_out
     c.stop = {"distance": _lib.JSV_STOP_DISTANCE, "rel_residual": _lib.JSV_STOP_REL_RESIDUAL}[stop]
     c.rtol = float(rtol)
     c.inner_cycles = int(inner_cycles)
-    c.check_every = 1
+    c.check_every = int(check_every)
     if history is not None:
         c.history = history.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         c.history_len = history.size
@@ -181,7 +181,7 @@ def validate_inputs(dim, phi, rhs, lam, D, D_xy):
 def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12, gamma=1,
           fcycle=False, skip_pre=False, nu1=1, nu2=1, max_iters_outer=INT32_MAX,
           max_iters_inner=INT32_MAX, precision="fp64", stop="distance", rtol=1e-10,
-          inner_cycles=4, return_history=False, return_info=False):
+          inner_cycles=0, check_every=1, return_history=False, return_info=False):
     ms = [g.mx for g in grids]
     levels, m_level = level_layout(ms)                  # Solver.__init__ -> compute_shapes
     check_grids(grids)                                  # compute_idel
@@ -223,7 +223,7 @@ def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12
         cap = max_iters_outer if (fixed or max_iters_outer < 100000) else 100000
         hist = np.full(int(min(cap, 10_000_000)), np.nan)
     cfg = make_config(fixed, epsilon, gamma, fcycle, skip_pre, nu1, nu2, max_iters_outer, stop,
-                      rtol, inner_cycles, hist)
+                      rtol, inner_cycles, hist, check_every)
     out = torch.empty_like(phi_d)
     res = _lib.JsvResult()
     _lib.check(_lib.load().jsv_solve(plan.h, ptr(phi_d), ptr(rhs_d), ptr(out), ctypes.byref(cfg),
@@ -282,7 +282,7 @@ def _solve_differentiable(dim, plan, dev, full_shape, phi, rhs, lam, D, D_xy, fi
 def solve_batched_2d(grid_x, grid_y, phi, rhs, lam, D_xx, D_yy, *, D_xy=None, fixed=False,
                      epsilon=1e-12, gamma=1, fcycle=False, skip_pre=False, nu1=1, nu2=1,
                      max_iters_outer=INT32_MAX, precision="fp64", stop="distance", rtol=1e-10,
-                     inner_cycles=4):
+                     inner_cycles=0, check_every=1):
     """B independent 2D problems on one grid, solved together (one launch per
     operation for the whole batch).  phi, rhs and array coefficients have shape
     (B, my+3, mx+3); lam / D may be Python floats shared by all problems.
@@ -315,7 +315,7 @@ def solve_batched_2d(grid_x, grid_y, phi, rhs, lam, D_xx, D_yy, *, D_xy=None, fi
     Dxy_v = arr_or_scalar(D_xy) if D_xy is not None else None
     plan.set_operator(lam_v, D_v, Dxy_v)
     cfg = make_config(fixed, epsilon, gamma, fcycle, skip_pre, nu1, nu2, max_iters_outer, stop, rtol,
-                      inner_cycles)
+                      inner_cycles, check_every=check_every)
     out = torch.empty_like(phi_d)
     res = _lib.JsvResult()
     counts = np.zeros(B, dtype=np.int64)

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -90,6 +91,8 @@ inline dim3 grid_flat(long n)
 
 }  // namespace
 
+constexpr int ACTK_ROWS = 64;        // cycles per staged block of per-problem activity rows
+
 struct jsv_plan {
     int dim = 0, prec = 0, levels = 0;
     int batch = 1;                  // 2D: independent problems solved together
@@ -105,6 +108,8 @@ struct jsv_plan {
     double* hscal = nullptr;        // pinned host mirror
     int* d_act = nullptr;           // per-problem "still iterating" flags
     int* h_act = nullptr;
+    int* d_actk = nullptr;          // per-cycle activity rows of a cycle block (ACTK_ROWS x batch)
+    int* h_actk = nullptr;          // pinned staging of d_actk
     long slab_bytes = 0;            // slab size of the 3D two-launch smoothing schedule (0: whole level)
     long two_pass_min = 1L << 20;   // level size from which restriction runs in two passes
     int fused_min_m = 256;          // 3D levels with mz >= this use the fused out-of-place sweep
@@ -112,6 +117,15 @@ struct jsv_plan {
     int sweep_rows = 0;             // rows per workgroup tile of the fused sweep (0: by precision)
     int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024)
     bool use_graphs = true;         // replay whole cycles as captured hipGraphs
+    int coarse_m = 0;               // levels with max resolution <= this run as one coarse program (0: off)
+    void* lv_dev[2] = {nullptr, nullptr};          // LvPtr<double> / LvPtr<float> per level
+    std::vector<const void*> lv_host[2];           // pointers lv_dev holds (start level + table)
+    struct CoarseProg {
+        int prec, gamma, nu1, nu2, fcyc, start, offdiag;
+        jsv::CoarseOp* ops;
+        int nops;
+    };
+    std::vector<CoarseProg> progs;
     hipStream_t cap_stream = nullptr;   // private stream for graph capture
     struct Graph {
         int prec, gamma, fcycle, skip_pre, nu1, nu2, use_act;
@@ -208,7 +222,7 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
 template <typename T>
 int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
 {
-    int rows = P->sweep_rows > 0 ? P->sweep_rows : (sizeof(T) == 4 ? 16 : 8);
+    int rows = P->sweep_rows > 0 ? P->sweep_rows : (sizeof(T) == 4 ? 16 : 4);
     const int nt = P->sweep_threads > 0 ? P->sweep_threads : 1024;
     if (nt == 256) {
         if (rows == 4) return launch_rbsweep3_t<T, 4, 256>(P, H, l, s);
@@ -343,6 +357,118 @@ int zero(T* a, long n, hipStream_t s)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// coarse-level program (k_coarse_prog): the sub-cycle below level `start - 1`
+// runs as one launch of one workgroup per problem
+// ---------------------------------------------------------------------------
+template <typename T>
+bool uses_fused(const jsv_plan* P, const Hier<T>& H, int l)
+{
+    return P->dim == 3 && P->fused_min_m > 0 && H.g[l].mz >= P->fused_min_m && H.phi2[l] != nullptr;
+}
+
+// first level (>= 1) handled by the coarse program, or -1
+template <typename T>
+int coarse_start(const jsv_plan* P, const Hier<T>& H)
+{
+    if (P->coarse_m <= 0) return -1;
+    for (int l = 1; l <= P->levels - 2; ++l) {
+        const Geo& g = H.g[l];
+        int mm = g.mx > g.my ? g.mx : g.my;
+        if (P->dim == 3 && g.mz > mm) mm = g.mz;
+        if (mm > P->coarse_m) continue;
+        for (int k = l; k < P->levels; ++k)
+            if (uses_fused(P, H, k)) return -1;
+        return l;
+    }
+    return -1;
+}
+
+// the op list of mg_cycle(l, fcyc, skip_pre = false), in mg_cycle's order
+void build_ops(const jsv_plan* P, const jsv_config& c, int l, bool fcyc, std::vector<jsv::CoarseOp>& ops)
+{
+    auto smooth_ops = [&](int lv) {
+        if (P->dim == 3) {
+            ops.push_back({jsv::CO_RB, lv, 1, 0});
+            ops.push_back({jsv::CO_RB, lv, 0, 0});
+        } else if (P->offdiag) {
+            static const int order[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+            for (int k = 0; k < 4; ++k) ops.push_back({jsv::CO_RB_DIAG, lv, order[k][0], order[k][1]});
+        } else {
+            ops.push_back({jsv::CO_RB, lv, 0, 0});
+            ops.push_back({jsv::CO_RB, lv, 1, 0});
+        }
+    };
+    if (l == P->levels - 2) {
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) smooth_ops(l);
+        return;
+    }
+    for (int i = 0; i < c.nu1; ++i) smooth_ops(l);
+    ops.push_back({jsv::CO_RESTRICT, l, 0, 0});
+    ops.push_back({jsv::CO_ZERO, l + 1, 0, 0});
+    if (fcyc)
+        for (int i = 0; i < c.gamma; ++i) build_ops(P, c, l + 1, true, ops);
+    for (int i = 0; i < c.gamma; ++i) build_ops(P, c, l + 1, false, ops);
+    const int full = (c.nu2 <= 0 || (P->dim == 2 && P->offdiag)) ? 1 : 0;
+    ops.push_back({jsv::CO_PROLONG, l, full, 0});
+    for (int i = 0; i < c.nu2; ++i) smooth_ops(l);
+}
+
+template <typename T>
+const jsv_plan::CoarseProg* find_prog(const jsv_plan* P, const jsv_config& c, bool fcyc, int start)
+{
+    const int prec = sizeof(T) == 4 ? JSV_MIXED : JSV_FP64;
+    for (const auto& pr : P->progs)
+        if (pr.prec == prec && pr.gamma == c.gamma && pr.nu1 == c.nu1 && pr.nu2 == c.nu2 &&
+            pr.fcyc == (fcyc ? 1 : 0) && pr.start == start && pr.offdiag == (P->offdiag ? 1 : 0))
+            return &pr;
+    return nullptr;
+}
+
+// Uploads the level table and the two op lists a cycle of configuration c
+// needs.  Synchronous copies: never called while a stream is being captured.
+template <typename T>
+int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
+{
+    const int start = coarse_start(P, H);
+    if (start < 0) return 0;
+    const int pi = sizeof(T) == 4 ? 1 : 0;
+    // only levels >= start are read by the program (levels above may swap
+    // their phi buffers every sweep)
+    // D_xy arrays only while the operator has them: they stay allocated (with
+    // stale values) after the plan switches back to a 5-point operator
+    auto drr = [&](int l) -> T* { return P->offdiag ? H.drr[l] : nullptr; };
+    auto dlr = [&](int l) -> T* { return P->offdiag ? H.dlr[l] : nullptr; };
+    std::vector<const void*> key{(const void*)(long)start};
+    for (int l = start; l < P->levels; ++l)
+        for (const T* q : {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l)})
+            key.push_back(q);
+    if (P->lv_host[pi] != key) {
+        std::vector<jsv::LvPtr<T>> lv(P->levels);
+        for (int l = start; l < P->levels; ++l)
+            lv[l] = {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l), H.g[l]};
+        const size_t nb = sizeof(jsv::LvPtr<T>) * lv.size();
+        if (P->lv_dev[pi] == nullptr) RETCHK(dalloc(P, &P->lv_dev[pi], nb));
+        HIPCHK(hipStreamSynchronize(s));   // a queued cycle may still read the table
+        HIPCHK(hipMemcpy(P->lv_dev[pi], lv.data(), nb, hipMemcpyHostToDevice));
+        P->lv_host[pi] = key;
+    }
+    for (int fc = 0; fc < 2; ++fc) {
+        if (find_prog<T>(P, c, fc != 0, start) != nullptr) continue;
+        std::vector<jsv::CoarseOp> ops;
+        ops.push_back({jsv::CO_ZERO, start, 0, 0});
+        if (fc)
+            for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, true, ops);
+        for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, false, ops);
+        jsv::CoarseOp* d = nullptr;
+        RETCHK(dalloc(P, (void**)&d, sizeof(jsv::CoarseOp) * ops.size()));
+        HIPCHK(hipMemcpy(d, ops.data(), sizeof(jsv::CoarseOp) * ops.size(), hipMemcpyHostToDevice));
+        P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
+                            P->offdiag ? 1 : 0, d, (int)ops.size()});
+    }
+    return 0;
+}
+
 // multigrid_cycle (solver_2d.py:582-614 / solver_3d.py:588-620)
 template <typename T>
 int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcyc, bool skip_pre,
@@ -355,10 +481,20 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
     if (!skip_pre)
         for (int i = 0; i < c.nu1; ++i) RETCHK(smooth(P, H, l, s));
     RETCHK(restrict_(P, H, l, s));
-    RETCHK(zero(H.phi[l + 1], H.g[l + 1].n, s));
-    if (fcyc)
-        for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, true, false, s));
-    for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, false, false, s));
+    const int start = coarse_start(P, H);
+    const jsv_plan::CoarseProg* pr = (start == l + 1) ? find_prog<T>(P, c, fcyc, start) : nullptr;
+    if (pr != nullptr) {
+        // zero + the gamma (F: 2 gamma) sub-cycles of level l + 1 in one launch
+        const int pi = sizeof(T) == 4 ? 1 : 0;
+        hipLaunchKernelGGL(jsv::k_coarse_prog<T>, dim3((unsigned)(P->dim == 2 ? P->batch : 1)),
+                           dim3(jsv::COARSE_THREADS), 0, s, (const jsv::LvPtr<T>*)P->lv_dev[pi],
+                           pr->ops, pr->nops, P->dim, P->act_ptr());
+    } else {
+        RETCHK(zero(H.phi[l + 1], H.g[l + 1].n, s));
+        if (fcyc)
+            for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, true, false, s));
+        for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, false, false, s));
+    }
     RETCHK(prolong(P, H, l, c.nu2, s));
     for (int i = 0; i < c.nu2; ++i) RETCHK(smooth(P, H, l, s));
     return 0;
@@ -382,6 +518,7 @@ template <typename T>
 int run_cycle(jsv_plan* P, Hier<T>& H, const jsv_config& c, bool skip_pre, hipStream_t s)
 {
     const int prec = sizeof(T) == 4 ? JSV_MIXED : JSV_FP64;
+    RETCHK(prepare_coarse(P, H, c, s));
     if (!P->use_graphs) return mg_cycle(P, H, c, 0, c.fcycle != 0, skip_pre, s);
     const int ua = P->use_act ? 1 : 0;
     for (auto& gr : P->graphs)
@@ -554,6 +691,7 @@ int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double i
     P->batch = batch;
     P->prec = precision;
     P->levels = L;
+    P->coarse_m = dim == 3 ? 16 : 64;
     for (int a = 0; a < 3; ++a) P->m[a] = ms[a];
     P->idel[0] = idel_x;
     P->idel[1] = idel_y;
@@ -569,9 +707,12 @@ int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double i
     }
     if (rc == 0) rc = dalloc(P, (void**)&P->dscal, sizeof(double) * 4 * batch);
     if (rc == 0) rc = dalloc(P, (void**)&P->d_act, sizeof(int) * batch);
+    if (rc == 0) rc = dalloc(P, (void**)&P->d_actk, sizeof(int) * ACTK_ROWS * batch);
     if (rc == 0) {
         hipError_t e = hipHostMalloc((void**)&P->hscal, sizeof(double) * 4 * batch, hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc((void**)&P->h_act, sizeof(int) * batch, hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&P->h_actk, sizeof(int) * ACTK_ROWS * batch, hipHostMallocDefault);
         if (e != hipSuccess) rc = fail("hipHostMalloc failed: %s", hipGetErrorString(e));
     }
     if (rc != 0) {
@@ -598,6 +739,7 @@ int jsv_plan_destroy(jsv_plan* P)
     for (void* p : P->allocs) (void)hipFree(p);
     if (P->hscal) (void)hipHostFree(P->hscal);
     if (P->h_act) (void)hipHostFree(P->h_act);
+    if (P->h_actk) (void)hipHostFree(P->h_actk);
     delete P;
     return 0;
 }
@@ -663,6 +805,10 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         };
         RETCHK(realloc_phi2(P->h64));
         if (P->prec == JSV_MIXED) RETCHK(realloc_phi2(P->h32));
+        return 0;
+    }
+    if (key == JSV_TUNE_COARSE_M) {
+        P->coarse_m = (int)value;
         return 0;
     }
     if (key == JSV_TUNE_ZCHUNK) {
@@ -885,9 +1031,11 @@ int jsv_cycle(jsv_plan* P, int prec, const jsv_config* cfg, int skip_pre, void* 
         if (P->prec != JSV_MIXED) return fail("plan has no FP32 hierarchy");
         // rim of the FP32 level-0 phi: zero via the same kernel on a temp view is
         // not available for float; the FP32 buffers are only written on unknowns
+        RETCHK(prepare_coarse(P, P->h32, *cfg, s));
         RETCHK(mg_cycle(P, P->h32, *cfg, 0, cfg->fcycle != 0, skip_pre != 0, s));
     } else {
         RETCHK(zero_rim_level0(P, P->h64.phi[0], s));
+        RETCHK(prepare_coarse(P, P->h64, *cfg, s));
         RETCHK(mg_cycle(P, P->h64, *cfg, 0, cfg->fcycle != 0, skip_pre != 0, s));
     }
     LAUNCHCHK();
@@ -984,6 +1132,22 @@ static void record(const jsv_config* c, int64_t k, double v)
 // after the first cycle whose stop quantity is not above the threshold (NaN
 // stops it too, as `dist > eps` is false for NaN in the reference); kernels of
 // later cycles skip it through the device flags.
+// Cycles until the relative residual is predicted to reach rtol, from the
+// contraction measured over the last k_last cycles (rel_prev -> rel_now);
+// clamped to [1, kmax].  Used to space residual checks (FP64, check_every = 0)
+// and FP64 refinement steps (mixed, inner_cycles = 0).
+static int64_t predict_cycles(double rel_prev, double rel_now, int64_t k_last, double rtol,
+                              int64_t kfirst, int64_t kmax)
+{
+    if (k_last <= 0 || !(rel_prev > 0.0) || !(rel_now > 0.0) || !std::isfinite(rel_now)) return kfirst;
+    const double rate = std::pow(rel_now / rel_prev, 1.0 / (double)k_last);
+    if (!(rate < 0.999999)) return kmax;
+    const double need = std::log(rtol / rel_now) / std::log(rate);
+    if (!(need > 1.0)) return 1;
+    const double k = std::ceil(need);
+    return k > (double)kmax ? kmax : (int64_t)k;
+}
+
 struct Track {
     std::vector<int64_t> count, outer;
     std::vector<double> last, rel;
@@ -1004,6 +1168,39 @@ static int push_act(jsv_plan* P, hipStream_t s)
 {
     HIPCHK(hipMemcpyAsync(P->d_act, P->h_act, sizeof(int) * P->batch, hipMemcpyHostToDevice, s));
     return 0;
+}
+
+// Runs `rows` cycles in which problem b takes part in the first kb[b] only
+// (its own schedule, so a batched solve gives each problem the bits of its
+// single solve).  Batched plans stage one activity row per cycle and copy a
+// row into the flags the kernels read whenever it changes; the outer flags
+// are restored afterwards.
+extern "C++" template <typename T>
+static int run_cycles_sched(jsv_plan* P, Hier<T>& H, const jsv_config& c, bool skip_pre,
+                           const std::vector<int64_t>& kb, int64_t rows, hipStream_t s)
+{
+    const int B = P->batch;
+    if (!P->use_act) {
+        for (int64_t j = 0; j < rows; ++j) RETCHK(run_cycle(P, H, c, skip_pre, s));
+        return 0;
+    }
+    for (int64_t j0 = 0; j0 < rows; j0 += ACTK_ROWS) {
+        const int64_t nr = rows - j0 < ACTK_ROWS ? rows - j0 : ACTK_ROWS;
+        if (j0 > 0) HIPCHK(hipStreamSynchronize(s));   // the staging buffer is reused
+        for (int64_t j = 0; j < nr; ++j)
+            for (int b = 0; b < B; ++b)
+                P->h_actk[j * B + b] = (P->h_act[b] && j0 + j < kb[b]) ? 1 : 0;
+        HIPCHK(hipMemcpyAsync(P->d_actk, P->h_actk, sizeof(int) * nr * B, hipMemcpyHostToDevice, s));
+        for (int64_t j = 0; j < nr; ++j) {
+            const bool same = (j > 0) && std::memcmp(P->h_actk + j * B, P->h_actk + (j - 1) * B,
+                                                     sizeof(int) * B) == 0;
+            if (!same)
+                HIPCHK(hipMemcpyAsync(P->d_act, P->d_actk + j * B, sizeof(int) * B,
+                                      hipMemcpyDeviceToDevice, s));
+            RETCHK(run_cycle(P, H, c, skip_pre, s));
+        }
+    }
+    return push_act(P, s);
 }
 
 static void fill_result(const Track& T, jsv_result* res, int64_t* counts, double* stopvals, int B)
@@ -1060,36 +1257,59 @@ static int solve_fp64(jsv_plan* P, const double* phi_in, const double* rhs, doub
         const double thr = relres ? c->rtol : c->epsilon;
         for (int b = 0; b < B; ++b)
             if (c->max_iters <= 0) { P->h_act[b] = 0; --T.active; }
-        int64_t it = 0;
+        if (T.active < B) RETCHK(push_act(P, s));
+        // residual checks every cycle (reference-like exact count) or spaced by
+        // the predicted cycles-to-tolerance (check_every = 0, rel_residual
+        // only).  Each problem keeps its own check schedule: every active
+        // problem runs every cycle, and at a check only the problems due are
+        // judged, as in their single solves.
+        const bool adaptive = relres && c->check_every == 0;
+        const int64_t every = relres ? (c->check_every > 0 ? c->check_every : 1) : 1;
+        const int64_t first_block = adaptive ? 8 : every;
+        std::vector<int64_t> due(B), last_chk(B, 0), nchk(B, 0);
+        std::vector<double> rel_prev(B, -1.0);
+        for (int b = 0; b < B; ++b) due[b] = first_block < c->max_iters ? first_block : c->max_iters;
+        int64_t done = 0;
         while (T.active > 0 && rc == 0) {
-            const double* old = (it == 0) ? first_old : P->phi_old;
-            if (!relres && it > 0)
+            const double* old = (done == 0) ? first_old : P->phi_old;
+            if (!relres && done > 0)
                 HIPCHK(hipMemcpyAsync(P->phi_old, H.phi[0], sizeof(double) * n,
                                       hipMemcpyDeviceToDevice, s));
-            rc = run_cycle(P, H, *c, c->skip_pre != 0, s);
+            int64_t next = INT64_MAX;
+            for (int b = 0; b < B; ++b)
+                if (P->h_act[b] && due[b] < next) next = due[b];
+            int64_t k = next - done;
+            if (k < 1) k = 1;
+            for (int64_t j = 0; j < k && rc == 0; ++j) rc = run_cycle(P, H, *c, c->skip_pre != 0, s);
             if (rc) break;
+            done += k;
             LAUNCHCHK();
             if (relres) RETCHK(residual64<double>(P, H.phi[0], H.rhs[0], (double*)nullptr, 0, s));
             else RETCHK(maxdiff(P, old, H.phi[0], 0, s));
             RETCHK(read_scalars(P, 1, s));
             bool changed = false;
             for (int b = 0; b < B; ++b) {
-                if (!P->h_act[b]) continue;
+                if (!P->h_act[b] || due[b] > done) continue;
                 const double v = relres ? std::sqrt(P->hscal[b]) / rhs_norm[b] : P->hscal[b];
                 T.last[b] = v;
                 if (relres) T.rel[b] = v;
                 if (!std::isfinite(v)) T.nonfinite[b] = 1;
-                if (b == 0) record(c, T.count[0], v);
-                ++T.count[b];
-                T.outer[b] = T.count[b];
-                if (!(v > thr) || T.count[b] >= c->max_iters) {
+                T.count[b] = T.outer[b] = done;
+                if (b == 0) record(c, done - 1, v);
+                if (!(v > thr) || done >= c->max_iters) {
                     P->h_act[b] = 0;
                     --T.active;
                     changed = true;
+                    continue;
                 }
+                int64_t kb = every;
+                if (adaptive) kb = predict_cycles(rel_prev[b], v, nchk[b] == 0 ? 0 : done - last_chk[b], thr, 8, 64);
+                rel_prev[b] = v;
+                last_chk[b] = done;
+                ++nchk[b];
+                due[b] = done + kb < c->max_iters ? done + kb : c->max_iters;
             }
             if (changed && T.active > 0) RETCHK(push_act(P, s));
-            ++it;
         }
     }
     P->use_act = false;
@@ -1112,7 +1332,12 @@ static int solve_mixed(jsv_plan* P, const double* phi_in, const double* rhs, dou
     const Geo& g = H.g[0];
     const long n = g.n;
     const int B = nseg(P);
-    const int k_inner = c->inner_cycles > 0 ? c->inner_cycles : 4;
+    // FP32 cycles per refinement step: fixed, or (inner_cycles = 0) predicted
+    // from the measured contraction so steps land near the tolerance
+    // per problem, as in its single solve
+    const bool adaptive = c->inner_cycles <= 0;
+    std::vector<double> rel_prev(B, -1.0);
+    std::vector<int64_t> k_last(B, 0), kb(B, 0);
     HIPCHK(hipMemcpyAsync(H.rhs[0], rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(H.phi[0], phi_in, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     RETCHK(zero_rim_level0(P, H.phi[0], s));
@@ -1154,12 +1379,20 @@ static int solve_mixed(jsv_plan* P, const double* phi_in, const double* rhs, dou
         first = false;
         if (T.active == 0) break;
         if (changed) RETCHK(push_act(P, s));
+        int64_t todo = 0;
+        for (int b = 0; b < B; ++b) {
+            kb[b] = 0;
+            if (!P->h_act[b]) continue;
+            int64_t k = c->inner_cycles;
+            if (adaptive) k = relres ? predict_cycles(rel_prev[b], T.rel[b], k_last[b], thr, 8, 64) : 8;
+            if (T.count[b] + k > c->max_iters) k = c->max_iters - T.count[b];
+            kb[b] = k;
+            todo = k > todo ? k : todo;
+        }
+        for (int b = 0; b < B; ++b) rel_prev[b] = T.rel[b];
         // inner FP32 cycles on the correction, from a zero start
         RETCHK(zero(F.phi[0], n, s));
-        int64_t todo = k_inner;
-        for (int b = 0; b < B; ++b)
-            if (P->h_act[b] && T.count[b] + todo > c->max_iters) todo = c->max_iters - T.count[b];
-        for (int64_t k = 0; k < todo && rc == 0; ++k) rc = run_cycle(P, F, *c, false, s);
+        rc = run_cycles_sched(P, F, *c, false, kb, todo, s);
         if (rc) break;
         if (!relres)
             HIPCHK(hipMemcpyAsync(P->phi_old, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -1168,7 +1401,8 @@ static int solve_mixed(jsv_plan* P, const double* phi_in, const double* rhs, dou
         LAUNCHCHK();
         for (int b = 0; b < B; ++b)
             if (P->h_act[b]) {
-                T.count[b] += todo;
+                T.count[b] += kb[b];
+                k_last[b] = kb[b];
                 ++T.outer[b];
             }
         if (!relres) {

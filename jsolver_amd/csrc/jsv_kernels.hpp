@@ -45,6 +45,52 @@ constexpr int BY = 4;    // 4 waves per workgroup
 // ghosts are never written and stay zero, which is what the reference's
 // do_BCs between sub-sweeps guarantees.
 // ---------------------------------------------------------------------------
+// Per-point updates (shared by the launch-per-half kernels below and the
+// single-workgroup coarse-level program kernel), in the reference's term order.
+template <typename T>
+__device__ __forceinline__ void rb2_point(T* __restrict__ phi, const T* __restrict__ rhs,
+                                          const T* __restrict__ cx, const T* __restrict__ cy,
+                                          const T* __restrict__ cc, long p, long sy)
+{
+    T acc = cy[p] * phi[p + sy] + cy[p - sy] * phi[p - sy];
+    acc = acc + cx[p] * phi[p + 1];
+    acc = acc + cx[p - 1] * phi[p - 1];
+    acc = acc - rhs[p];
+    phi[p] = acc / cc[p];
+}
+
+template <typename T>
+__device__ __forceinline__ void rb2diag_point(T* __restrict__ phi, const T* __restrict__ rhs,
+                                              const T* __restrict__ cx, const T* __restrict__ cy,
+                                              const T* __restrict__ cc, const T* __restrict__ drr,
+                                              const T* __restrict__ dlr, long p, long sy)
+{
+    T acc = cy[p] * phi[p + sy] + cy[p - sy] * phi[p - sy];
+    acc = acc + cx[p] * phi[p + 1];
+    acc = acc + cx[p - 1] * phi[p - 1];
+    acc = acc - rhs[p];
+    T off = drr[p - sy - 1] * phi[p - sy - 1] + dlr[p - sy + 1] * phi[p - sy + 1];
+    off = off + dlr[p] * phi[p + sy - 1];
+    off = off + drr[p] * phi[p + sy + 1];
+    acc = acc + off;
+    phi[p] = acc / cc[p];
+}
+
+template <typename T>
+__device__ __forceinline__ void rb3_point(T* __restrict__ phi, const T* __restrict__ rhs,
+                                          const T* __restrict__ cx, const T* __restrict__ cy,
+                                          const T* __restrict__ cz, const T* __restrict__ cc, long p,
+                                          long sy, long sz)
+{
+    T acc = cz[p] * phi[p + sz] + cz[p - sz] * phi[p - sz];
+    acc = acc + cy[p] * phi[p + sy];
+    acc = acc + cy[p - sy] * phi[p - sy];
+    acc = acc + cx[p] * phi[p + 1];
+    acc = acc + cx[p - 1] * phi[p - 1];
+    acc = acc - rhs[p];
+    phi[p] = acc / cc[p];
+}
+
 // 2D kernels: blockIdx.z is the problem of a batch; act (nullable) holds a
 // per-problem "still iterating" flag.
 template <typename T>
@@ -58,12 +104,7 @@ k_rb2(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
     if (y > g.my) return;
     const int x = 2 + ((par + y) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
     if (x > g.mx) return;
-    const long p = (long)b * g.sz + (long)y * g.sy + x;
-    T acc = cy[p] * phi[p + g.sy] + cy[p - g.sy] * phi[p - g.sy];
-    acc = acc + cx[p] * phi[p + 1];
-    acc = acc + cx[p - 1] * phi[p - 1];
-    acc = acc - rhs[p];
-    phi[p] = acc / cc[p];
+    rb2_point(phi, rhs, cx, cy, cc, (long)b * g.sz + (long)y * g.sy + x, g.sy);
 }
 
 // 2D with D_xy: the 9-point stencil couples the sub-lattices diagonally, so
@@ -80,17 +121,7 @@ k_rb2_diag(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__
     if (y > g.my) return;
     const int x = (lx ? 2 : 3) + 2 * (blockIdx.x * BX + threadIdx.x);
     if (x > g.mx) return;
-    const long p = (long)b * g.sz + (long)y * g.sy + x;
-    const long sy = g.sy;
-    T acc = cy[p] * phi[p + sy] + cy[p - sy] * phi[p - sy];
-    acc = acc + cx[p] * phi[p + 1];
-    acc = acc + cx[p - 1] * phi[p - 1];
-    acc = acc - rhs[p];
-    T off = drr[p - sy - 1] * phi[p - sy - 1] + dlr[p - sy + 1] * phi[p - sy + 1];
-    off = off + dlr[p] * phi[p + sy - 1];
-    off = off + drr[p] * phi[p + sy + 1];
-    acc = acc + off;
-    phi[p] = acc / cc[p];
+    rb2diag_point(phi, rhs, cx, cy, cc, drr, dlr, (long)b * g.sz + (long)y * g.sy + x, g.sy);
 }
 
 // z0: first plane of the launch (grid.z planes from z0); the slab schedule
@@ -106,14 +137,7 @@ k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
     if (y > g.my) return;
     const int x = 2 + ((par + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
     if (x > g.mx) return;
-    const long p = (long)z * g.sz + (long)y * g.sy + x;
-    T acc = cz[p] * phi[p + g.sz] + cz[p - g.sz] * phi[p - g.sz];
-    acc = acc + cy[p] * phi[p + g.sy];
-    acc = acc + cy[p - g.sy] * phi[p - g.sy];
-    acc = acc + cx[p] * phi[p + 1];
-    acc = acc + cx[p - 1] * phi[p - 1];
-    acc = acc - rhs[p];
-    phi[p] = acc / cc[p];
+    rb3_point(phi, rhs, cx, cy, cz, cc, (long)z * g.sz + (long)y * g.sy + x, g.sy, g.sz);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,39 +338,26 @@ __device__ __forceinline__ T resid3(const TP* __restrict__ phi, const TP* __rest
 // ghosts are not written (zero from allocation).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(BX * BY)
-k_restrict2(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
-            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cc,
-            const T* __restrict__ drr, const T* __restrict__ dlr, Geo gf, Geo gc,
-            const int* __restrict__ act)
+__device__ __forceinline__ T restrict2_at(const T* __restrict__ phi, const T* __restrict__ rhs,
+                                          const T* __restrict__ cx, const T* __restrict__ cy,
+                                          const T* __restrict__ cc, const T* __restrict__ drr,
+                                          const T* __restrict__ dlr, long c, long sy)
 {
-    const int b = blockIdx.z;
-    if (act != nullptr && act[b] == 0) return;
-    const int J = 2 + blockIdx.y * BY + threadIdx.y;
-    const int K = 2 + blockIdx.x * BX + threadIdx.x;
-    if (J > gc.my || K > gc.mx) return;
-    const long sy = gf.sy;
-    const long c = (long)b * gf.sz + (long)(2 * J - 1) * sy + (2 * K - 1);
+    // c: the fine point 2I-1 under coarse unknown I (problem offset included)
     T s = resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c + sy + 1, sy)
         + resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c + sy - 1, sy);
     s = s + resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c - sy + 1, sy);
     s = s + resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c - sy - 1, sy);
     const T rcen = resid2<T, T>(phi, rhs, cx, cy, cc, drr, dlr, c, sy);
-    rc[(long)b * gc.sz + (long)J * gc.sy + K] = (T)0.0625 * s + (T)0.25 * rcen;
+    return (T)0.0625 * s + (T)0.25 * rcen;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BX * BY)
-k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
-            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-            const T* __restrict__ cc, Geo gf, Geo gc)
+__device__ __forceinline__ T restrict3_at(const T* __restrict__ phi, const T* __restrict__ rhs,
+                                          const T* __restrict__ cx, const T* __restrict__ cy,
+                                          const T* __restrict__ cz, const T* __restrict__ cc, long c,
+                                          long sy, long sz)
 {
-    const int I = 2 + blockIdx.z;
-    const int J = 2 + blockIdx.y * BY + threadIdx.y;
-    const int K = 2 + blockIdx.x * BX + threadIdx.x;
-    if (J > gc.my || K > gc.mx) return;
-    const long sy = gf.sy, sz = gf.sz;
-    const long c = (long)(2 * I - 1) * sz + (long)(2 * J - 1) * sy + (2 * K - 1);
 #define JSV_R(o) resid3<T, T>(phi, rhs, cx, cy, cz, cc, c + (o), sy, sz)
     T e = JSV_R(sy + 1) + JSV_R(sy - 1);
     e = e + JSV_R(-sy + 1);
@@ -361,7 +372,37 @@ k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__
     e = e + JSV_R(-sz - sy);
     const T rcen = JSV_R(0);
 #undef JSV_R
-    rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * rcen;
+    return (T)0.03125 * e + (T)0.125 * rcen;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_restrict2(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
+            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cc,
+            const T* __restrict__ drr, const T* __restrict__ dlr, Geo gf, Geo gc,
+            const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int J = 2 + blockIdx.y * BY + threadIdx.y;
+    const int K = 2 + blockIdx.x * BX + threadIdx.x;
+    if (J > gc.my || K > gc.mx) return;
+    const long c = (long)b * gf.sz + (long)(2 * J - 1) * gf.sy + (2 * K - 1);
+    rc[(long)b * gc.sz + (long)J * gc.sy + K] = restrict2_at(phi, rhs, cx, cy, cc, drr, dlr, c, gf.sy);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
+            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+            const T* __restrict__ cc, Geo gf, Geo gc)
+{
+    const int I = 2 + blockIdx.z;
+    const int J = 2 + blockIdx.y * BY + threadIdx.y;
+    const int K = 2 + blockIdx.x * BX + threadIdx.x;
+    if (J > gc.my || K > gc.mx) return;
+    const long c = (long)(2 * I - 1) * gf.sz + (long)(2 * J - 1) * gf.sy + (2 * K - 1);
+    rc[(long)I * gc.sz + (long)J * gc.sy + K] = restrict3_at(phi, rhs, cx, cy, cz, cc, c, gf.sy, gf.sz);
 }
 
 // Two-pass restriction for large levels: pass 1 stores the residual at every
@@ -448,22 +489,15 @@ k_restrict_gather2(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc,
 // i/2 and i/2+1.  The interpolations the reference skips when nu2 > 0 (points
 // of the first RBGS half, overwritten before they are read) are skipped alike.
 // ---------------------------------------------------------------------------
+// Correction at fine point (x, y[, z]) from coarse array pc (problem offset
+// included); false where the point is skipped (full == 0).
 template <typename T>
-__global__ void __launch_bounds__(BX * BY)
-k_prolong2(T* __restrict__ phi, const T* __restrict__ pcb, Geo gf, Geo gc, int full,
-           const int* __restrict__ act)
+__device__ __forceinline__ bool prolong2_corr(const T* __restrict__ pc, long csy, int x, int y,
+                                              int full, T& corr)
 {
-    const int b = blockIdx.z;
-    if (act != nullptr && act[b] == 0) return;
-    const int y = 2 + blockIdx.y * BY + threadIdx.y;
-    const int x = 2 + blockIdx.x * BX + threadIdx.x;
-    if (y > gf.my || x > gf.mx) return;
-    const T* __restrict__ pc = pcb + (long)b * gc.sz;
-    const long csy = gc.sy;
     const int ox = x & 1, oy = y & 1;
     const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Xl = x >> 1, Yl = y >> 1;
 #define C2(J, K) pc[(long)(J) * csy + (K)]
-    T corr;
     if (ox && oy) {
         corr = C2(Yo, Xo);
     } else if (oy) {
@@ -471,32 +505,25 @@ k_prolong2(T* __restrict__ phi, const T* __restrict__ pcb, Geo gf, Geo gc, int f
     } else if (ox) {
         corr = (T)0.5 * (C2(Yl + 1, Xo) + C2(Yl, Xo));
     } else {
-        if (!full) return;
+        if (!full) return false;
         T s = C2(Yl + 1, Xl + 1) + C2(Yl, Xl + 1);
         s = s + C2(Yl + 1, Xl);
         s = s + C2(Yl, Xl);
         corr = (T)0.25 * s;
     }
 #undef C2
-    const long p = (long)b * gf.sz + (long)y * gf.sy + x;
-    phi[p] = phi[p] + corr;
+    return true;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BX * BY)
-k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int full)
+__device__ __forceinline__ bool prolong3_corr(const T* __restrict__ pc, long csy, long csz, int x,
+                                              int y, int z, int full, T& corr)
 {
-    const int z = 2 + blockIdx.z;
-    const int y = 2 + blockIdx.y * BY + threadIdx.y;
-    const int x = 2 + blockIdx.x * BX + threadIdx.x;
-    if (y > gf.my || x > gf.mx) return;
-    const long csy = gc.sy, csz = gc.sz;
     const int ox = x & 1, oy = y & 1, oz = z & 1;
     const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Zo = (z + 1) >> 1;
     const int Xl = x >> 1, Yl = y >> 1, Zl = z >> 1;
 #define C3(I, J, K) pc[(long)(I) * csz + (long)(J) * csy + (K)]
     const int nodd = ox + oy + oz;
-    T corr;
     if (nodd == 3) {
         corr = C3(Zo, Yo, Xo);
     } else if (nodd == 2) {
@@ -504,7 +531,7 @@ k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int fu
         else if (!oy) corr = (T)0.5 * (C3(Zo, Yl + 1, Xo) + C3(Zo, Yl, Xo));
         else          corr = (T)0.5 * (C3(Zl + 1, Yo, Xo) + C3(Zl, Yo, Xo));
     } else if (nodd == 1) {
-        if (!full) return;
+        if (!full) return false;
         T s;
         if (oz) {
             s = C3(Zo, Yl + 1, Xl + 1) + C3(Zo, Yl, Xl + 1);
@@ -531,8 +558,171 @@ k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int fu
         corr = (T)0.125 * s;
     }
 #undef C3
+    return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_prolong2(T* __restrict__ phi, const T* __restrict__ pcb, Geo gf, Geo gc, int full,
+           const int* __restrict__ act)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    if (y > gf.my || x > gf.mx) return;
+    T corr;
+    if (!prolong2_corr(pcb + (long)b * gc.sz, gc.sy, x, y, full, corr)) return;
+    const long p = (long)b * gf.sz + (long)y * gf.sy + x;
+    phi[p] = phi[p] + corr;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int full)
+{
+    const int z = 2 + blockIdx.z;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    if (y > gf.my || x > gf.mx) return;
+    T corr;
+    if (!prolong3_corr(pc, gc.sy, gc.sz, x, y, z, full, corr)) return;
     const long p = (long)z * gf.sz + (long)y * gf.sy + x;
     phi[p] = phi[p] + corr;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-level program.  Below ~16^3 (3D) / 64^2 (2D) unknowns a level's
+// smoothing half, restriction or prolongation is a few thousand points: as
+// separate launches each costs a launch plus a fill/drain of the whole GPU for
+// microseconds of work.  One workgroup per problem instead runs the whole
+// coarse sub-cycle from a host-built op list, with a workgroup barrier between
+// ops (the coarse levels stay resident in L2).  The per-point arithmetic is
+// the same device functions the launch-per-op kernels use, so results are
+// bit-identical.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LvPtr {
+    T* phi;
+    T* rhs;
+    const T* cx;
+    const T* cy;
+    const T* cz;
+    const T* cc;
+    const T* drr;
+    const T* dlr;
+    Geo g;
+};
+
+enum CoarseOpKind : int {
+    CO_RB = 0,        // one RBGS half: a = parity
+    CO_RB_DIAG = 1,   // one D_xy sub-colour: a = lx, b = ly
+    CO_RESTRICT = 2,  // defect of `level` restricted into rhs of level + 1
+    CO_ZERO = 3,      // phi of `level` := 0 (whole array incl. rim)
+    CO_PROLONG = 4,   // phi of `level` += P phi of level + 1; a = full
+};
+
+struct CoarseOp {
+    int kind, level, a, b;
+};
+
+constexpr int COARSE_THREADS = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(COARSE_THREADS)
+k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops, int nops, int dim,
+              const int* __restrict__ act)
+{
+    const int b = blockIdx.x;               // 2D: problem of the batch; 3D: 0
+    if (act != nullptr && act[b] == 0) return;
+    const int tid = threadIdx.x;
+    for (int o = 0; o < nops; ++o) {
+        const CoarseOp op = ops[o];
+        const LvPtr<T> L = lv[op.level];
+        const Geo& g = L.g;
+        const long base = dim == 2 ? (long)b * g.sz : 0;
+        if (op.kind == CO_RB) {
+            const int xh = g.mx / 2, ny = g.my - 1;
+            if (dim == 3) {
+                const int cnt = xh * ny * (g.mz - 1);
+                for (int i = tid; i < cnt; i += COARSE_THREADS) {
+                    const int z = 2 + i / (xh * ny), r = i % (xh * ny);
+                    const int y = 2 + r / xh;
+                    const int x = 2 + ((op.a + y + z) & 1) + 2 * (r % xh);
+                    if (x > g.mx) continue;
+                    rb3_point(L.phi, L.rhs, L.cx, L.cy, L.cz, L.cc,
+                              (long)z * g.sz + (long)y * g.sy + x, g.sy, g.sz);
+                }
+            } else {
+                const int cnt = xh * ny;
+                for (int i = tid; i < cnt; i += COARSE_THREADS) {
+                    const int y = 2 + i / xh;
+                    const int x = 2 + ((op.a + y) & 1) + 2 * (i % xh);
+                    if (x > g.mx) continue;
+                    rb2_point(L.phi, L.rhs, L.cx, L.cy, L.cc, base + (long)y * g.sy + x, g.sy);
+                }
+            }
+        } else if (op.kind == CO_RB_DIAG) {
+            const int xh = g.mx / 2, yh = g.my / 2;
+            for (int i = tid; i < xh * yh; i += COARSE_THREADS) {
+                const int y = (op.b ? 2 : 3) + 2 * (i / xh);
+                const int x = (op.a ? 2 : 3) + 2 * (i % xh);
+                if (y > g.my || x > g.mx) continue;
+                rb2diag_point(L.phi, L.rhs, L.cx, L.cy, L.cc, L.drr, L.dlr,
+                              base + (long)y * g.sy + x, g.sy);
+            }
+        } else if (op.kind == CO_RESTRICT) {
+            const LvPtr<T> C = lv[op.level + 1];
+            const Geo& gc = C.g;
+            const int nx = gc.mx - 1, ny = gc.my - 1;
+            if (dim == 3) {
+                const int cnt = nx * ny * (gc.mz - 1);
+                for (int i = tid; i < cnt; i += COARSE_THREADS) {
+                    const int I = 2 + i / (nx * ny), r = i % (nx * ny);
+                    const int J = 2 + r / nx, K = 2 + r % nx;
+                    const long c = (long)(2 * I - 1) * g.sz + (long)(2 * J - 1) * g.sy + (2 * K - 1);
+                    C.rhs[(long)I * gc.sz + (long)J * gc.sy + K] =
+                        restrict3_at(L.phi, L.rhs, L.cx, L.cy, L.cz, L.cc, c, g.sy, g.sz);
+                }
+            } else {
+                const long cbase = (long)b * gc.sz;
+                for (int i = tid; i < nx * ny; i += COARSE_THREADS) {
+                    const int J = 2 + i / nx, K = 2 + i % nx;
+                    const long c = base + (long)(2 * J - 1) * g.sy + (2 * K - 1);
+                    C.rhs[cbase + (long)J * gc.sy + K] =
+                        restrict2_at(L.phi, L.rhs, L.cx, L.cy, L.cc, L.drr, L.dlr, c, g.sy);
+                }
+            }
+        } else if (op.kind == CO_ZERO) {
+            const long cnt = dim == 2 ? g.sz : g.n;
+            for (long i = tid; i < cnt; i += COARSE_THREADS) L.phi[base + i] = (T)0;
+        } else {  // CO_PROLONG
+            const LvPtr<T> C = lv[op.level + 1];
+            const Geo& gc = C.g;
+            const int nx = g.mx - 1, ny = g.my - 1;
+            if (dim == 3) {
+                const int cnt = nx * ny * (g.mz - 1);
+                for (int i = tid; i < cnt; i += COARSE_THREADS) {
+                    const int z = 2 + i / (nx * ny), r = i % (nx * ny);
+                    const int y = 2 + r / nx, x = 2 + r % nx;
+                    T corr;
+                    if (!prolong3_corr(C.phi, gc.sy, gc.sz, x, y, z, op.a, corr)) continue;
+                    const long p = (long)z * g.sz + (long)y * g.sy + x;
+                    L.phi[p] = L.phi[p] + corr;
+                }
+            } else {
+                const long cbase = (long)b * gc.sz;
+                for (int i = tid; i < nx * ny; i += COARSE_THREADS) {
+                    const int y = 2 + i / nx, x = 2 + i % nx;
+                    T corr;
+                    if (!prolong2_corr(C.phi + cbase, gc.sy, x, y, op.a, corr)) continue;
+                    const long p = base + (long)y * g.sy + x;
+                    L.phi[p] = L.phi[p] + corr;
+                }
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------

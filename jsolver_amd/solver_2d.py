@@ -10,7 +10,10 @@ are never modified.
 Keyword-only additions to ``solve_2d`` (no counterpart in the reference):
   precision="fp64" | "mixed"  -- FP32 cycles inside FP64 iterative refinement
   stop="distance" | "rel_residual", rtol=1e-10 -- stop on ||rhs - L phi|| / ||rhs||
-  inner_cycles=4      -- FP32 cycles per refinement step ("mixed")
+  inner_cycles=0      -- FP32 cycles per refinement step ("mixed"; 0 = predicted
+                         from the measured contraction)
+  check_every=1       -- "fp64" + rel_residual: cycles between residual checks
+                         (0 = spaced by the predicted cycles-to-tolerance)
   return_history=False -- also return the per-cycle stop quantity (numpy)
   return_info=False   -- also return a dict (count, outer steps, last distance ...)
 """

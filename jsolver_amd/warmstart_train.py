@@ -72,9 +72,8 @@ def generate(samples, m, batch, seed, label_rtol, dev, log=print):
         X.append(torch.stack([torch.log(dxx[sl]), torch.log(dyy[sl]), lam[sl], rhs[sl]], -1).float())
         Y.append(phi[sl].float())
         counts.append(cnt)
-        if (i // batch) % 20 == 0:
-            log(f"[data] {i + B}/{samples} solved ({time.perf_counter() - t0:.1f}s, "
-                f"mean W-cycles {np.mean(cnt):.1f})")
+        log(f"[data] {i + B}/{samples} solved ({time.perf_counter() - t0:.1f}s, "
+            f"mean W-cycles {np.mean(cnt):.1f})")
     return torch.cat(X), torch.cat(Y), np.concatenate(counts)
 
 

@@ -189,6 +189,53 @@ def test_cycle_bitexact(case, cyc):
     np.testing.assert_array_equal(got, want)
 
 
+COARSE_CASES = [c for c in CASES if c[0] in ("2d_spatial_16x32", "2d_dxy_16", "3d_spatial_16x8x32",
+                                               "3d_scalar_lamarr")] + [
+    ("2d_spatial_128x64", 2, (128, 64), True, True, False),
+    ("2d_dxy_64", 2, (64, 64), True, True, True),
+    ("3d_spatial_32", 3, (32, 32, 32), True, True, False),
+]
+
+
+@pytest.mark.parametrize("coarse_m", [0, 4, 1 << 20])
+@pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4]], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", COARSE_CASES, ids=[c[0] for c in COARSE_CASES])
+def test_cycle_coarse_program_bitexact(case, cyc, coarse_m):
+    """The single-workgroup coarse program (off, from the 4-point levels, and
+    from level 1 down) gives the oracle's bits for whole cycles."""
+    from jsolver_amd import _lib
+    dp, op, phi, rhs = _setup(case, seed=17)
+    _lib.check(_lib.load().jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, coarse_m))
+    kw = cyc[1]
+    c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+    for rep in range(2):                      # second call replays the cached program
+        dp.upload(0, 0, phi)
+        dp.upload(0, 1, rhs)
+        dp.cycle(cfg(**kw))
+        want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
+        np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"call {rep}")
+
+
+def test_cycle_coarse_program_operator_switch():
+    """One plan, operator switched D_xy -> 5-point -> D_xy: each cycle still
+    gives the oracle's bits (the D_xy arrays stay allocated in between)."""
+    ms = (64, 64)
+    phi, rhs, lam, D, Dxy = rand_problem(2, ms, seed=23, dxy=True)
+    dp = DevPlan(2, ms, lam, D, Dxy)
+    dev = dp.plan.dev
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    for kw in (dict(), dict(gamma=2), dict(fcycle=True)):
+        for use_dxy in (True, False, True):
+            dp.plan.set_operator(t(lam), [t(d) for d in D], t(Dxy) if use_dxy else None)
+            op = oracle_operator(2, ms, lam, D, Dxy if use_dxy else None)
+            c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+            dp.upload(0, 0, phi)
+            dp.upload(0, 1, rhs)
+            dp.cycle(cfg(**kw))
+            np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle),
+                                          err_msg=f"{kw} dxy={use_dxy}")
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_residual_matches_oracle(case):
     dp, op, phi, rhs = _setup(case, seed=7)
@@ -376,9 +423,15 @@ def test_do_bcs_3d():
 # batched 2D solves: every problem equals its own solve (bit for bit)
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("kw", [dict(), dict(stop="rel_residual", rtol=1e-9), dict(gamma=2, fcycle=True),
+                                dict(stop="rel_residual", rtol=1e-9, check_every=0),
+                                dict(stop="rel_residual", rtol=1e-9, check_every=3, max_iters_outer=40),
                                 dict(precision="mixed", stop="rel_residual", rtol=1e-10),
+                                dict(precision="mixed", stop="rel_residual", rtol=1e-10, inner_cycles=5,
+                                     max_iters_outer=33),
+                                dict(precision="mixed", gamma=2),
                                 dict(fixed=True, max_iters_outer=5)],
-                         ids=["distance", "relres", "W_F", "mixed", "fixed"])
+                         ids=["distance", "relres", "W_F", "relres_adaptive", "relres_every3_cap",
+                              "mixed", "mixed_inner5_cap", "mixed_distance", "fixed"])
 def test_batched_equals_single(kw):
     from jsolver_amd.solver_2d import solve_batched_2d
     ms = (32, 32)
@@ -398,5 +451,5 @@ def test_batched_equals_single(kw):
     for b in range(B):
         assert counts[b] == singles[b][1], b
         np.testing.assert_array_equal(out[b].cpu().numpy(), singles[b][0].cpu().numpy())
-    if not kw.get("fixed"):
+    if "max_iters_outer" not in kw:
         assert len(set(counts.tolist())) > 1      # the problems really stopped at different cycles
