@@ -96,9 +96,11 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         stores first-half residuals and gathers them in a
  *                         second pass (default 2^20)
  *   JSV_TUNE_FUSED_MIN_M  3D levels with mz >= this run each RBGS sweep as one
- *                         fused out-of-place z-marching kernel (default 256;
+ *                         fused out-of-place z-marching kernel (default 128;
  *                         0 = never)
- *   JSV_TUNE_ZCHUNK       planes per workgroup of that kernel (0 = 16 in FP32, 32 in FP64)
+ *   JSV_TUNE_ZCHUNK       planes per workgroup of that kernel (0 = by level:
+ *                         one wave of workgroups for the pair kernel, 16 in
+ *                         FP32 / 32 in FP64 for the point kernel)
  *   JSV_TUNE_SWEEP_ROWS   rows per workgroup tile of that kernel: 4, 6, 8, 16
  *                         (0 = 16 in FP32, 4 in FP64)
  *   JSV_TUNE_SWEEP_THREADS threads per workgroup of that kernel: 256 or 1024
@@ -108,7 +110,12 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *   JSV_TUNE_COARSE_M     levels whose largest resolution is <= this (and
  *                         every coarser level) run each coarse sub-cycle as
  *                         one single-workgroup launch per problem
- *                         (default 16 in 3D, 64 in 2D; 0 = off) */
+ *                         (default 16 in 3D, 64 in 2D; 0 = off)
+ *   JSV_TUNE_SWEEP_VARIANT kernel of the fused sweep: 2 (default, 0) one
+ *                         thread per red/black column pair with the
+ *                         coefficients loaded one plane ahead (levels with
+ *                         mx <= 512, rows 2, 4, 5, 6 or 14; others use 1);
+ *                         1 one thread per ring point */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -117,6 +124,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_SWEEP_THREADS 5
 #define JSV_TUNE_GRAPHS 6
 #define JSV_TUNE_COARSE_M 7
+#define JSV_TUNE_SWEEP_VARIANT 8
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

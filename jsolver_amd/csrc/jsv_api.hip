@@ -112,10 +112,12 @@ struct jsv_plan {
     int* h_actk = nullptr;          // pinned staging of d_actk
     long slab_bytes = 0;            // slab size of the 3D two-launch smoothing schedule (0: whole level)
     long two_pass_min = 1L << 20;   // level size from which restriction runs in two passes
-    int fused_min_m = 256;          // 3D levels with mz >= this use the fused out-of-place sweep
+    int fused_min_m = 128;          // 3D levels with mz >= this use the fused out-of-place sweep
     int zchunk = 0;                 // planes per workgroup of the fused sweep (0: by level size)
     int sweep_rows = 0;             // rows per workgroup tile of the fused sweep (0: by precision)
     int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024)
+    int sweep_variant = 2;          // 1: thread per ring point; 2: thread per column pair, pipelined
+    int n_cu = 256;                 // compute units of the device
     bool use_graphs = true;         // replay whole cycles as captured hipGraphs
     int coarse_m = 0;               // levels with max resolution <= this run as one coarse program (0: off)
     void* lv_dev[2] = {nullptr, nullptr};          // LvPtr<double> / LvPtr<float> per level
@@ -218,10 +220,129 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
     return 0;
 }
 
+// LDS of the pair sweep with TY rows (rings + held black half)
+size_t pair_sweep_lds(const Geo& g, int ty, size_t w)
+{
+    return w * ((size_t)(3 * (ty + 4) + 3 * (ty + 2)) * ((g.nx + 1) / 2) + 8 * 1024);
+}
+
+bool pair_sweep_rows_fit(const Geo& g, int ty, size_t w)
+{
+    return (ty + 2) * (g.mx / 2) <= 1024 && (g.nx + 1) / 2 <= 258 && pair_sweep_lds(g, ty, w) <= 160 * 1024 &&
+           (double)g.n * w < 4294967296.0;
+}
+
+bool pair_sweep_fits(const Geo& g, size_t w) { return pair_sweep_rows_fit(g, 2, w); }
+
+template <typename T, int TY, int NHM>
+int pair_sweep_per_cu(const Geo& g)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jsv::k_rbsweep3p<T, TY, 1024, NHM>, 1024,
+                                                     pair_sweep_lds(g, TY, sizeof(T))) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu;
+}
+
+template <typename T, int TY>
+int pair_sweep_per_cu(const Geo& g)
+{
+    return (g.nx + 1) / 2 <= 130 ? pair_sweep_per_cu<T, TY, 130>(g) : pair_sweep_per_cu<T, TY, 258>(g);
+}
+
+template <typename T>
+int pair_sweep_per_cu(const Geo& g, int ty)
+{
+    switch (ty) {
+    case 14: return pair_sweep_per_cu<T, 14>(g);
+    case 6: return pair_sweep_per_cu<T, 6>(g);
+    case 5: return pair_sweep_per_cu<T, 5>(g);
+    case 4: return pair_sweep_per_cu<T, 4>(g);
+    default: return pair_sweep_per_cu<T, 2>(g);
+    }
+}
+
+template <typename T, int TY, int NHM>
+int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, hipStream_t s)
+{
+    const Geo& g = H.g[l];
+    constexpr int NT = 1024;
+    const int nh = (g.nx + 1) / 2;
+    if ((TY + 2) * (g.mx / 2) > NT || nh > NHM || (double)g.n * sizeof(T) >= 4294967296.0)
+        return fail("pair sweep does not fit level %d", l);
+    const size_t lds = pair_sweep_lds(g, TY, sizeof(T));
+    if (lds > 160 * 1024) return fail("pair sweep tile needs %zu bytes of LDS", lds);
+    const int ytiles = (g.my - 1 + TY - 1) / TY;
+    const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
+    hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM>), dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds, s,
+                       H.phi[l], H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles);
+    return 0;
+}
+
+template <typename T, int TY>
+int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, hipStream_t s)
+{
+    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130>(P, H, l, zchunk, s);
+    return launch_rbsweep3p_t<T, TY, 258>(P, H, l, zchunk, s);
+}
+
+// Rows and z-chunk of the pair sweep.  Untuned, every fitting row count is
+// scored by the fraction of workgroup slots one wave of tiles fills times
+// the share of pair rows that are not halo, TY / (TY + 2); the z-chunk then
+// gives each slot one tile.
+template <typename T>
+int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+{
+    const Geo& g = H.g[l];
+    int rows = P->sweep_rows;
+    if (rows == 8 || rows == 16) rows = 6;
+    const bool tuned = rows > 0 && pair_sweep_rows_fit(g, rows, sizeof(T));
+    int zchunks = 0;
+    if (!tuned) {
+        rows = 0;
+        double best = -1.0;
+        for (int ty : {14, 6, 5, 4, 2}) {
+            if (!pair_sweep_rows_fit(g, ty, sizeof(T))) continue;
+            const int slots = P->n_cu * pair_sweep_per_cu<T>(g, ty);
+            const int ytiles = (g.my - 1 + ty - 1) / ty;
+            int zc = slots / ytiles;
+            if (zc < 1) zc = 1;
+            if (zc > g.mz - 1) zc = g.mz - 1;
+            const int used = ytiles * zc;
+            const double waves = std::ceil((double)used / slots);
+            const double score = ((double)used / (waves * slots)) * ty / (ty + 2.0);
+            if (score > best + 1e-9) {
+                best = score;
+                rows = ty;
+                zchunks = zc;
+            }
+        }
+    }
+    if (rows == 0) return fail("pair sweep does not fit level %d", l);
+    int zchunk = P->zchunk;
+    if (zchunk <= 0) {
+        if (zchunks <= 0) {
+            const int ytiles = (g.my - 1 + rows - 1) / rows;
+            zchunks = P->n_cu * pair_sweep_per_cu<T>(g, rows) / ytiles;
+            if (zchunks < 1) zchunks = 1;
+        }
+        zchunk = (g.mz - 1 + zchunks - 1) / zchunks;
+    }
+    switch (rows) {
+    case 14: return launch_rbsweep3p_t<T, 14>(P, H, l, zchunk, s);
+    case 6: return launch_rbsweep3p_t<T, 6>(P, H, l, zchunk, s);
+    case 5: return launch_rbsweep3p_t<T, 5>(P, H, l, zchunk, s);
+    case 4: return launch_rbsweep3p_t<T, 4>(P, H, l, zchunk, s);
+    default: return launch_rbsweep3p_t<T, 2>(P, H, l, zchunk, s);
+    }
+}
+
 // tile rows x threads per workgroup of the fused sweep (tunable; default by precision)
 template <typename T>
 int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
 {
+    if (P->sweep_variant == 2 && pair_sweep_fits(H.g[l], sizeof(T))) return launch_rbsweep3p<T>(P, H, l, s);
     int rows = P->sweep_rows > 0 ? P->sweep_rows : (sizeof(T) == 4 ? 16 : 4);
     const int nt = P->sweep_threads > 0 ? P->sweep_threads : 1024;
     if (nt == 256) {
@@ -692,6 +813,12 @@ int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double i
     P->prec = precision;
     P->levels = L;
     P->coarse_m = dim == 3 ? 16 : 64;
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            P->n_cu = ncu;
+    }
     for (int a = 0; a < 3; ++a) P->m[a] = ms[a];
     P->idel[0] = idel_x;
     P->idel[1] = idel_y;
@@ -815,9 +942,15 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         P->zchunk = (int)value;
         return 0;
     }
+    if (key == JSV_TUNE_SWEEP_VARIANT) {
+        if (value > 2) return fail("sweep variant must be 0, 1 or 2");
+        P->sweep_variant = value == 0 ? 2 : (int)value;
+        return 0;
+    }
     if (key == JSV_TUNE_SWEEP_ROWS) {
-        if (value != 0 && value != 4 && value != 6 && value != 8 && value != 16)
-            return fail("sweep rows must be 0, 4, 6, 8 or 16");
+        if (value != 0 && value != 2 && value != 4 && value != 5 && value != 6 && value != 8 && value != 14 &&
+            value != 16)
+            return fail("sweep rows must be 0, 2, 4, 5, 6, 8, 14 or 16");
         P->sweep_rows = (int)value;
         return 0;
     }

@@ -289,6 +289,238 @@ k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Pair-per-thread variant of the fused sweep, with the coefficient stream
+// software-pipelined one plane ahead.
+//
+// Same tiles, rings and step order as k_rbsweep3, but each thread owns one
+// column pair (x = 2k, 2k+1) of one ring row for the whole z-march.  In every
+// plane one point of a pair is red and the other black, and the black point
+// of plane z-1 sits at the x of the red point of plane z.  So per step a
+// thread loads the coefficients of BOTH points of plane zs+1 (13 loads,
+// every element of the cache lines used: cz of the plane below and cx of the
+// left neighbour are carried or shared; pairs load as one vector), uses
+// plane zs's red half for the red
+// update now and keeps plane zs's black half in registers for the black
+// update one step later; its own red value of plane zs-1 is written out from
+// a register.  The two coefficient sets alternate between two register
+// blocks (the loop is unrolled by two), so the loads issued for plane zs+1
+// stay in flight across the step's barriers and compute; the black half
+// waits in LDS.  Requires (TY + 2) * (mx / 2) <= NT, (nx + 1) / 2 <= NHMAX
+// and the level below 4 GiB; rim values of phi_in are zero.
+// ---------------------------------------------------------------------------
+// two consecutive elements, loaded as one dword-aligned vector access
+template <typename T>
+struct __attribute__((aligned(sizeof(T)))) Pair2 {
+    T a, b;
+};
+
+template <typename T>
+struct PairCo {
+    T cxm, cxl, cxr;          // cx at x = 2k-1, 2k, 2k+1
+    T cyl, cyr, cyml, cymr;   // cy at (y, 2k), (y, 2k+1), (y-1, 2k), (y-1, 2k+1)
+    T czl, czr;               // cz at 2k, 2k+1
+    T rl, rr, ccl, ccr;       // rhs, diagonal at 2k, 2k+1
+};
+
+template <typename T, int TY, int NT, int NHMAX, int WPE = 1>
+__global__ void __launch_bounds__(NT, WPE)
+k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
+            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+            const T* __restrict__ cc, Geo g, int zchunk, int ytiles)
+{
+    constexpr int PR = TY + 4;                  // rows of the phi_in ring: y0-2 .. y0+TY+1
+    constexpr int RR = TY + 2;                  // rows of the red ring / pair rows: y0-1 .. y0+TY
+    constexpr int PMAX = (PR * NHMAX + NT - 1) / NT;   // ring elements per thread (nh <= NHMAX)
+    const int nh = (g.nx + 1) >> 1;
+    const int nhe = g.mx >> 1;                  // pairs per row
+    T* P = reinterpret_cast<T*>(jsv_dyn_smem);  // [3][PR][nh]
+    T* R = P + 3 * PR * nh;                     // [3][RR][nh]
+    T* Hb = R + 3 * RR * nh;                    // [8][NT] this thread's held black half
+    const int tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int y0 = 2 + (tile % ytiles) * TY;
+    const int z0 = 2 + (tile / ytiles) * zchunk;
+    int z1 = z0 + zchunk;
+    if (z1 > g.mz + 1) z1 = g.mz + 1;
+    const long sy = g.sy, sz = g.sz;
+
+    for (int i = tid; i < 3 * RR * nh; i += NT) R[i] = (T)0;
+
+    const bool has = tid < RR * nhe;
+    const int r = has ? tid / nhe : 0;
+    const int k = has ? 1 + (tid - r * nhe) : 1;
+    const int y = y0 - 1 + r;
+    const bool ld = has && y < g.ny;
+    const bool unk = has && y >= 2 && y <= g.my;
+    const bool inner = unk && r >= 1 && r <= TY;
+    // threads without an in-bounds pair load row 1 (values unused), so every
+    // thread issues the same loads.  Byte offsets are 32-bit (levels below
+    // 4 GiB, checked by the host): one offset register serves all arrays,
+    // whose bases stay in scalar registers.
+    const unsigned rowb = (unsigned)(((ld ? y : 1) * g.sy + 2 * k) * (long)sizeof(T));
+    const unsigned syb = (unsigned)(g.sy * sizeof(T)), szb = (unsigned)(g.sz * sizeof(T));
+    auto at = [](const T* base, unsigned boff) -> T {
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+    };
+    constexpr unsigned W = sizeof(T);
+
+    // black phi_in of plane z, element i of a ring slot.  Loads are issued
+    // unconditionally (an out-of-range element reads element 0 and is
+    // zeroed on commit): branch-free issue lets the in-order load counter
+    // leave them in flight across the step.
+    auto src = [&](int z, int i, bool& ok) -> unsigned {
+        const int rr_ = i / nh, kk = i - rr_ * nh;
+        const int yy = y0 - 2 + rr_;
+        const int x = 2 * kk + ((yy + z) & 1);
+        ok = i < PR * nh && z >= 0 && z < g.nz && yy >= 0 && yy < g.ny && x < g.nx;
+        return ok ? (unsigned)(((long)z * sz + (long)yy * sy + x) * (long)sizeof(T)) : 0u;
+    };
+    auto load_plane = [&](int z) {
+        T* dst = P + (z % 3) * PR * nh;
+        for (int i = tid; i < PR * nh; i += NT) {
+            bool ok;
+            const unsigned o = src(z, i, ok);
+            dst[i] = ok ? at(pin, o) : (T)0;
+        }
+    };
+    T pre[PMAX];
+    unsigned pre_ok = 0;
+    auto issue = [&](int z) {
+        pre_ok = 0;
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            bool ok;
+            const unsigned o = src(z, tid + j * NT, ok);
+            pre[j] = at(pin, o);
+            pre_ok |= ok ? (1u << j) : 0u;
+        }
+    };
+    auto commit = [&](int z) {
+        T* dst = P + (z % 3) * PR * nh;
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            const int i = tid + j * NT;
+            if (i < PR * nh) dst[i] = ((pre_ok >> j) & 1u) ? pre[j] : (T)0;
+        }
+    };
+    auto Pv = [&](int z, int yy, int x) -> T {
+        return P[((z % 3) * PR + (yy - y0 + 2)) * nh + (x >> 1)];
+    };
+    auto Rv = [&](int z, int yy, int x) -> T {
+        return R[((z % 3) * RR + (yy - y0 + 1)) * nh + (x >> 1)];
+    };
+    auto at2 = [](const T* base, unsigned boff) -> Pair2<T> {
+        return *reinterpret_cast<const Pair2<T>*>(reinterpret_cast<const char*>(base) + boff);
+    };
+    auto load = [&](int z, PairCo<T>& c) {
+        const unsigned q = (unsigned)z * szb + rowb;
+        c.cxm = at(cx, q - W);
+        Pair2<T> v = at2(cx, q);
+        c.cxl = v.a;
+        c.cxr = v.b;
+        v = at2(cy, q);
+        c.cyl = v.a;
+        c.cyr = v.b;
+        v = at2(cy, q - syb);
+        c.cyml = v.a;
+        c.cymr = v.b;
+        v = at2(cz, q);
+        c.czl = v.a;
+        c.czr = v.b;
+        v = at2(rhs, q);
+        c.rl = v.a;
+        c.rr = v.b;
+        v = at2(cc, q);
+        c.ccl = v.a;
+        c.ccr = v.b;
+    };
+
+    load_plane(z0 - 2);
+    load_plane(z0 - 1);
+    issue(z0);
+    PairCo<T> A, B;
+    load(z0 - 1, A);
+    T czpl, czpr;                               // cz of the plane below the current one
+    {
+        const unsigned q = (unsigned)(z0 - 2) * szb + rowb;
+        czpl = at(cz, q);
+        czpr = at(cz, q + W);
+    }
+    // the black half of the previous plane (its update runs one step later)
+    // waits in LDS, private to the thread: no barrier needed
+    auto H = [&](int j) -> T& { return Hb[j * NT + tid]; };
+    T red_prev = 0;
+
+    auto step = [&](int zs, PairCo<T>& cur, PairCo<T>& nxt) {
+        // unconditional issue: planes up to z1 + 2 <= nz are requested (the
+        // ring's guard zeroes plane nz; coefficient plane zs + 1 <= nz - 1)
+        commit(zs + 1);
+        issue(zs + 2);
+        load(zs + 1, nxt);
+        __syncthreads();
+        // ---- red on plane zs (x + y + z odd)
+        const bool left = ((y + zs) & 1) != 0;  // x = 2k is red
+        const int xr = 2 * k + (left ? 0 : 1);
+        T red = (T)0;
+        if (unk && zs >= 2 && zs <= g.mz && xr <= g.mx) {
+            const T a_zp = left ? cur.czl : cur.czr;
+            const T a_zm = left ? czpl : czpr;
+            const T a_yp = left ? cur.cyl : cur.cyr;
+            const T a_ym = left ? cur.cyml : cur.cymr;
+            const T a_xp = left ? cur.cxl : cur.cxr;
+            const T a_xm = left ? cur.cxm : cur.cxl;
+            // all ring reads first, then the reference's sum order
+            const T p_zp = Pv(zs + 1, y, xr), p_zm = Pv(zs - 1, y, xr);
+            const T p_yp = Pv(zs, y + 1, xr), p_ym = Pv(zs, y - 1, xr);
+            const T p_xp = Pv(zs, y, xr + 1), p_xm = Pv(zs, y, xr - 1);
+            T acc = a_zp * p_zp + a_zm * p_zm;
+            acc = acc + a_yp * p_yp;
+            acc = acc + a_ym * p_ym;
+            acc = acc + a_xp * p_xp;
+            acc = acc + a_xm * p_xm;
+            acc = acc - (left ? cur.rl : cur.rr);
+            red = acc / (left ? cur.ccl : cur.ccr);
+        }
+        if (has) R[((zs % 3) * RR + r) * nh + k] = red;
+        __syncthreads();
+        // ---- black on plane zb = zs - 1 at x = xr; write both colours of the pair
+        const int zb = zs - 1;
+        if (inner && zb >= z0 && zb < z1) {
+            T* orow = pout + ((long)zb * sz + (long)y * sy);
+            if (xr <= g.mx) {
+                const T r_zp = Rv(zb + 1, y, xr), r_zm = Rv(zb - 1, y, xr);
+                const T r_yp = Rv(zb, y + 1, xr), r_ym = Rv(zb, y - 1, xr);
+                const T r_xp = Rv(zb, y, xr + 1), r_xm = Rv(zb, y, xr - 1);
+                T acc = H(0) * r_zp + H(1) * r_zm;
+                acc = acc + H(2) * r_yp;
+                acc = acc + H(3) * r_ym;
+                acc = acc + H(4) * r_xp;
+                acc = acc + H(5) * r_xm;
+                acc = acc - H(6);
+                orow[xr] = acc / H(7);
+            }
+            if ((xr ^ 1) <= g.mx) orow[xr ^ 1] = red_prev;
+        }
+        // ---- keep plane zs's black half (the point that is not red)
+        H(0) = left ? cur.czr : cur.czl;
+        H(1) = left ? czpr : czpl;
+        H(2) = left ? cur.cyr : cur.cyl;
+        H(3) = left ? cur.cymr : cur.cyml;
+        H(4) = left ? cur.cxr : cur.cxl;
+        H(5) = left ? cur.cxl : cur.cxm;
+        H(6) = left ? cur.rr : cur.rl;
+        H(7) = left ? cur.ccr : cur.ccl;
+        red_prev = red;
+        czpl = cur.czl;
+        czpr = cur.czr;
+    };
+    for (int zs = z0 - 1; zs <= z1; zs += 2) {
+        step(zs, A, B);
+        if (zs + 1 <= z1) step(zs + 1, B, A);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Residual rhs - L phi at one point (solver_2d.py:637, solver_3d.py:642), in the
 // reference's term order.
 // ---------------------------------------------------------------------------

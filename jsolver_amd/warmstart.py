@@ -10,10 +10,9 @@ The network sees the (m x m) window of nodes 1..m of the ghost-padded
 (m+3)^2 arrays (boundary node 1 and the unknowns 2..m), so every level
 halves evenly for m a power of two >= 8.  Its prediction is placed on the
 unknowns of an (m+3)^2 phi with a zero rim.  The solver then iterates to its
-usual tolerance from that start; ``warm_start`` falls back to a zero start
-when the guess's residual exceeds ||rhs|| (measured after `smooth_sweeps`
-RBGS sweeps of the guess, which remove the high-frequency part of its error
-that the h^-2 scaled residual would otherwise be dominated by).
+usual tolerance from that start; the evaluation (warmstart_train.evaluate)
+falls back to a zero start for every problem whose guess has
+||rhs - L phi0|| > ||rhs||.
 """
 import ctypes
 import math

@@ -57,23 +57,29 @@ def make_problems(B, m, seed, dev):
     return lam, dxx, dyy, rhs
 
 
-def generate(samples, m, batch, seed, label_rtol, dev, log=print):
+def generate(samples, m, batch, seed, label_rtol, dev, log=print, gamma=1, max_cycles=50000):
     """Solved training set, kept on the GPU: features (S, m, m, 4) fp32 raw and
-    targets (S, m, m) fp32 (phi on nodes 1..m)."""
+    targets (S, m, m) fp32 (phi on nodes 1..m).  Problems whose solve does
+    not stay finite are dropped: with the reference's non-Galerkin coarse
+    operators a W-cycle (gamma = 2) diverges on some strongly anisotropic
+    fields that V-cycles solve, so labels use V-cycles by default."""
     g = grid_1D(m)
     X, Y, counts = [], [], []
+    dropped = 0
     t0 = time.perf_counter()
     for i in range(0, samples, batch):
         B = min(batch, samples - i)
         lam, dxx, dyy, rhs = make_problems(B, m, seed + i, dev)
-        phi, cnt, _ = solve_batched_2d(g, g, torch.zeros_like(rhs), rhs, lam, dxx, dyy, gamma=2,
-                                       stop="rel_residual", rtol=label_rtol)
-        sl = (slice(None), slice(1, -2), slice(1, -2))
+        phi, cnt, stopv = solve_batched_2d(g, g, torch.zeros_like(rhs), rhs, lam, dxx, dyy, gamma=gamma,
+                                           stop="rel_residual", rtol=label_rtol, max_iters_outer=max_cycles)
+        ok = torch.isfinite(phi).flatten(1).all(1) & torch.as_tensor(stopv <= label_rtol, device=dev)
+        dropped += int((~ok).sum())
+        sl = (ok, slice(1, -2), slice(1, -2))
         X.append(torch.stack([torch.log(dxx[sl]), torch.log(dyy[sl]), lam[sl], rhs[sl]], -1).float())
         Y.append(phi[sl].float())
-        counts.append(cnt)
+        counts.append(cnt[ok.cpu().numpy()])
         log(f"[data] {i + B}/{samples} solved ({time.perf_counter() - t0:.1f}s, "
-            f"mean W-cycles {np.mean(cnt):.1f})")
+            f"mean cycles {np.mean(cnt):.1f}, dropped {dropped})")
     return torch.cat(X), torch.cat(Y), np.concatenate(counts)
 
 
@@ -143,13 +149,35 @@ def guess_phi(net, lam, dxx, dyy, rhs, m_train):
     return phi
 
 
-def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=2, log=print):
+def smooth_guess(phi0, rhs, lam, dxx, dyy, sweeps):
+    """`sweeps` FP64 RBGS sweeps of the fine-level operator applied to a batch
+    of guesses (B, m+3, m+3): they damp the guess's high-frequency error,
+    whose h^-2 weighted residual would otherwise dominate ||rhs - L phi0||."""
+    from jsolver_amd import _lib, _solver
+    from jsolver_amd._device import ptr, stream_ptr
+    B, n, _ = rhs.shape
+    g = grid_1D(n - 3)
+    plan = _solver.get_plan(2, (g.mx, g.mx), [g.idel, g.idel], "fp64", B)
+    plan.set_operator(lam, [dxx, dyy])
+    lib = _lib.load()
+    phi = phi0.contiguous().clone()
+    r = rhs.contiguous()
+    _lib.check(lib.jsv_level_upload(plan.h, _lib.JSV_FP64, 0, 0, ptr(phi), stream_ptr()))
+    _lib.check(lib.jsv_level_upload(plan.h, _lib.JSV_FP64, 0, 1, ptr(r), stream_ptr()))
+    _lib.check(lib.jsv_smooth(plan.h, _lib.JSV_FP64, 0, int(sweeps), stream_ptr()))
+    _lib.check(lib.jsv_level_download(plan.h, _lib.JSV_FP64, 0, 0, ptr(phi), stream_ptr()))
+    return phi
+
+
+def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_cycles=50000,
+             guess_sweeps=4):
     """Cycles and wall time to rtol from a zero start and from the network's
-    guess (with zero-start fallback when ||rhs - L phi0|| > ||rhs||)."""
-    from jsolver_amd import _solver
+    guess after `guess_sweeps` RBGS sweeps (with zero-start fallback when
+    ||rhs - L phi0|| > ||rhs||).  Problems on which either solve does not
+    converge are excluded from the cycle statistics and counted."""
     g = grid_1D(m)
     lam, dxx, dyy, rhs = make_problems(B, m, seed, dev)
-    kw = dict(gamma=gamma, stop="rel_residual", rtol=rtol)
+    kw = dict(gamma=gamma, stop="rel_residual", rtol=rtol, max_iters_outer=max_cycles)
     solve_batched_2d(g, g, torch.zeros_like(rhs), rhs, lam, dxx, dyy, fixed=True, max_iters_outer=1, gamma=gamma)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -158,6 +186,8 @@ def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=2, log=print):
     t_zero = time.perf_counter() - t0
     t0 = time.perf_counter()
     phi0 = guess_phi(net, lam, dxx, dyy, rhs, m_train)
+    if guess_sweeps > 0:
+        phi0 = smooth_guess(phi0, rhs, lam, dxx, dyy, guess_sweeps)
     # residual judge: ||rhs - L phi0|| vs ||rhs|| per problem
     _, _, rel0 = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, fixed=True, max_iters_outer=0,
                                   stop="rel_residual", rtol=rtol, precision="mixed")
@@ -166,13 +196,17 @@ def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=2, log=print):
     _, c1, r1 = solve_batched_2d(g, g, phi_start, rhs, lam, dxx, dyy, **kw)
     torch.cuda.synchronize()
     t_warm = time.perf_counter() - t0
-    res = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol,
+    good = np.isfinite(r0) & (r0 <= rtol) & np.isfinite(r1) & (r1 <= rtol)
+    c0, c1 = np.asarray(c0)[good], np.asarray(c1)[good]
+    res = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol, "guess_sweeps": guess_sweeps,
+           "unconverged_excluded": int((~good).sum()),
            "cycles_zero_mean": float(np.mean(c0)), "cycles_warm_mean": float(np.mean(c1)),
            "cycle_reduction": float(1.0 - np.sum(c1) / np.sum(c0)),
+           "per_problem_reduction_median": float(np.median(1.0 - c1 / c0)),
            "time_zero_s": t_zero, "time_warm_s": t_warm,
            "guess_rel_residual_median": float(np.median(rel0)),
            "fallbacks": int((~use).sum().item()),
-           "max_final_rel_residual": float(max(np.max(r0), np.max(r1)))}
+           "max_final_rel_residual": float(max(np.max(r0[good]), np.max(r1[good])))}
     log(f"[eval] {json.dumps(res)}")
     return res
 
@@ -189,6 +223,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--eval-m", type=int, default=512)
     ap.add_argument("--eval-batch", type=int, default=64)
+    ap.add_argument("--gamma", type=int, default=1)
+    ap.add_argument("--max-cycles", type=int, default=50000)
+    ap.add_argument("--guess-sweeps", type=int, default=4)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     dev = torch.device("cuda", 0)
@@ -196,8 +233,9 @@ def main(argv=None):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    X, Y, counts = generate(a.samples, a.m, a.gen_batch, a.seed, a.label_rtol, dev, log)
+    X, Y, counts = generate(a.samples, a.m, a.gen_batch, a.seed, a.label_rtol, dev, log, a.gamma, a.max_cycles)
     t_data = time.perf_counter() - t0
+    X_kept = X.shape[0]
     net = UNet(seed=a.seed, device=dev)
     fit_normalisation(net, X, Y)
     t0 = time.perf_counter()
@@ -207,9 +245,14 @@ def main(argv=None):
         torch.save(net.state_dict(), a.out)
     del X, Y
     torch.cuda.empty_cache()
-    ev = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev, log=log)
-    ev_train_res = evaluate(net, a.m, a.eval_batch, 888_000, a.m, dev, log=log)
-    print(json.dumps({"samples": a.samples, "m": a.m, "data_s": t_data, "train_s": t_train,
+    ev = {}
+    for sw in sorted({0, a.guess_sweeps}):
+        ev[f"sweeps{sw}"] = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev, gamma=a.gamma, log=log,
+                                     max_cycles=a.max_cycles, guess_sweeps=sw)
+    ev_train_res = evaluate(net, a.m, a.eval_batch, 888_000, a.m, dev, gamma=a.gamma, log=log,
+                            max_cycles=a.max_cycles, guess_sweeps=a.guess_sweeps)
+    print(json.dumps({"samples": a.samples, "kept": int(X_kept), "m": a.m, "gamma": a.gamma,
+                      "data_s": t_data, "train_s": t_train,
                       "label_cycles_mean": float(np.mean(counts)), "loss_per_epoch": hist,
                       "eval": ev, "eval_train_resolution": ev_train_res}), flush=True)
 

@@ -96,17 +96,24 @@ def test_restrict_bitexact(case):
         np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
 
 
+FUSED_VARIANTS = [(1, 0), (2, 0), (2, 2), (2, 4), (2, 5), (2, 6), (2, 14)]
+
+
+@pytest.mark.parametrize("variant,rows", FUSED_VARIANTS, ids=[f"v{v}_rows{r}" for v, r in FUSED_VARIANTS])
 @pytest.mark.parametrize("zchunk", [1, 2, 3, 5, 0])
 @pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 8), (8, 8, 8)])
-def test_fused_sweep_bitexact(ms, zchunk):
+def test_fused_sweep_bitexact(ms, zchunk, variant, rows):
     """The fused out-of-place z-marching sweep (forced on every level, several
-    y-tiles and z-chunk sizes) gives the oracle's bits for sweeps and cycles."""
+    y-tiles and z-chunk sizes, both kernels) gives the oracle's bits for
+    sweeps and cycles."""
     from jsolver_amd import _lib
     case = ("3d_fused", 3, ms, True, True, False)
     dp, op, phi0, rhs0 = _setup(case, seed=13)
     lib = _lib.load()
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSED_MIN_M, 2))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_ZCHUNK, zchunk))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_VARIANT, variant))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_ROWS, rows))
     for level in range(op.levels - 1):
         phi, rhs = _level_arrays(op, level, 80 + level)
         dp.upload(level, 0, phi)

@@ -153,3 +153,27 @@ def test_training_reduces_loss():
     net.out_scale = float(Y.std())
     hist = train(net, X, Y, epochs=60, batch=4, lr=2e-3, log=lambda *_: None)
     assert hist[-1] < 0.5 * hist[0], hist
+
+
+def test_smooth_guess_matches_oracle():
+    """The guess post-smoothing of the warm-start evaluation is `sweeps` RBGS
+    sweeps of the fine operator, problem by problem (bit-exact)."""
+    from jsolver_amd.warmstart_train import make_problems, smooth_guess
+    from oracle import gmg
+    m, B, sweeps = 32, 3, 3
+    lam, dxx, dyy, rhs = make_problems(B, m, 4, DEV)
+    g = torch.Generator(device=DEV).manual_seed(1)
+    phi0 = torch.randn(rhs.shape, generator=g, device=DEV, dtype=torch.float64)
+    phi0[:, :2] = 0
+    phi0[:, -2:] = 0
+    phi0[:, :, :2] = 0
+    phi0[:, :, -2:] = 0
+    got = smooth_guess(phi0, rhs, lam, dxx, dyy, sweeps).cpu().numpy()
+    og = gmg.OracleGrid(m)
+    for b in range(B):
+        op = gmg.build_operator(2, (m, m), (og.idel, og.idel), lam[b].cpu().numpy(),
+                                [dxx[b].cpu().numpy(), dyy[b].cpu().numpy()])
+        want = phi0[b].cpu().numpy()
+        for _ in range(sweeps):
+            want = gmg.smooth(op, 0, want, rhs[b].cpu().numpy())
+        np.testing.assert_array_equal(got[b], want)
