@@ -32,7 +32,8 @@ def parse():
                     help="FP32 cycles per refinement step (0: predicted from the contraction)")
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps per roofline timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-cycles", type=int, default=1)
+    ap.add_argument("--cpu-cycles", type=int, default=8,
+                    help="oracle V-cycles timed for the CPU baseline (about 1.7 s each at 256^3)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only set up the operator and time the per-level sweeps (for profiling)")
     return ap.parse_args()
@@ -72,7 +73,8 @@ def roofline(a, grids, m):
             if level == 0 and prec_name == "fp32":
                 roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": _traffic(),
-                        "kernel": "k_rb3<float>: one RBGS sweep (red + black launches), fine level",
+                        "kernel": "k_rbsweep3p<float>: one fused out-of-place RBGS sweep (red + black "
+                                  "halves, one launch), fine level",
                         "algorithmic_bytes_per_sweep": 7 * w * n_unk,
                         "us_per_sweep": round(t_sweep * 1e6, 2)}
     log("per-level sweeps: " + ", ".join(f"{p['prec']}/L{p['level']}:{p['GBps']}GB/s" for p in per_level))
@@ -80,14 +82,15 @@ def roofline(a, grids, m):
 
 
 def _traffic():
-    """HBM bytes per fine-level FP32 sweep measured with rocprofv3 PMC
-    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), recorded in
-    profiles/traffic_rb3_fine_256.json; None when absent."""
+    """HBM bytes per launch of the fine-level FP32 sweep kernel measured with
+    rocprofv3 PMC (FETCH_SIZE + WRITE_SIZE, each scaled by its reading on a
+    copy kernel of known size), recorded in
+    profiles/traffic_rbsweep3p_fp32_256.json; None when absent."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "traffic_rb3_fine_256.json")
+                        "traffic_rbsweep3p_fp32_256.json")
     try:
         with open(path) as f:
-            return json.load(f)["bytes_per_sweep"]
+            return json.load(f)["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 

@@ -170,11 +170,15 @@ def smooth_guess(phi0, rhs, lam, dxx, dyy, sweeps):
 
 
 def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_cycles=50000,
-             guess_sweeps=4):
+             guess_sweeps=4, judge="rhs"):
     """Cycles and wall time to rtol from a zero start and from the network's
-    guess after `guess_sweeps` RBGS sweeps (with zero-start fallback when
-    ||rhs - L phi0|| > ||rhs||).  Problems on which either solve does not
-    converge are excluded from the cycle statistics and counted."""
+    guess after `guess_sweeps` RBGS sweeps, with zero-start fallback per
+    problem.  judge="rhs": fall back when ||rhs - L phi0|| > ||rhs|| (the
+    residual of the zero start).  judge="cycle": run one cycle from the guess
+    and one from zero and fall back when the guess's residual after it is the
+    larger; that cycle pair is counted in the warm-start cycles.  Problems on
+    which either solve does not converge are excluded from the cycle
+    statistics and counted."""
     g = grid_1D(m)
     lam, dxx, dyy, rhs = make_problems(B, m, seed, dev)
     kw = dict(gamma=gamma, stop="rel_residual", rtol=rtol, max_iters_outer=max_cycles)
@@ -188,23 +192,37 @@ def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_
     phi0 = guess_phi(net, lam, dxx, dyy, rhs, m_train)
     if guess_sweeps > 0:
         phi0 = smooth_guess(phi0, rhs, lam, dxx, dyy, guess_sweeps)
-    # residual judge: ||rhs - L phi0|| vs ||rhs|| per problem
-    _, _, rel0 = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, fixed=True, max_iters_outer=0,
-                                  stop="rel_residual", rtol=rtol, precision="mixed")
-    use = torch.tensor(rel0 <= 1.0, device=dev)
+    extra = 0
+    if judge == "rhs":
+        # ||rhs - L phi0|| vs ||rhs|| per problem
+        _, _, rel0 = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, fixed=True, max_iters_outer=0,
+                                      stop="rel_residual", rtol=rtol, precision="mixed")
+        use = torch.tensor(rel0 <= 1.0, device=dev)
+    else:
+        # residuals after one cycle from the guess and from zero
+        one = dict(fixed=True, max_iters_outer=1, gamma=gamma)
+        pg, _, _ = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, **one)
+        pz, _, _ = solve_batched_2d(g, g, torch.zeros_like(rhs), rhs, lam, dxx, dyy, **one)
+        res0 = dict(fixed=True, max_iters_outer=0, stop="rel_residual", rtol=rtol, precision="mixed")
+        _, _, rg = solve_batched_2d(g, g, pg, rhs, lam, dxx, dyy, **res0)
+        _, _, rz = solve_batched_2d(g, g, pz, rhs, lam, dxx, dyy, **res0)
+        rel0 = rg / np.maximum(rz, 1e-300)
+        use = torch.tensor(rg <= rz, device=dev)
+        extra = 2
     phi_start = torch.where(use[:, None, None], phi0, torch.zeros_like(phi0))
     _, c1, r1 = solve_batched_2d(g, g, phi_start, rhs, lam, dxx, dyy, **kw)
+    c1 = np.asarray(c1) + extra
     torch.cuda.synchronize()
     t_warm = time.perf_counter() - t0
     good = np.isfinite(r0) & (r0 <= rtol) & np.isfinite(r1) & (r1 <= rtol)
     c0, c1 = np.asarray(c0)[good], np.asarray(c1)[good]
-    res = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol, "guess_sweeps": guess_sweeps,
+    res = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol, "guess_sweeps": guess_sweeps, "judge": judge,
            "unconverged_excluded": int((~good).sum()),
            "cycles_zero_mean": float(np.mean(c0)), "cycles_warm_mean": float(np.mean(c1)),
            "cycle_reduction": float(1.0 - np.sum(c1) / np.sum(c0)),
            "per_problem_reduction_median": float(np.median(1.0 - c1 / c0)),
            "time_zero_s": t_zero, "time_warm_s": t_warm,
-           "guess_rel_residual_median": float(np.median(rel0)),
+           "guess_judge_ratio_median": float(np.median(rel0)),
            "fallbacks": int((~use).sum().item()),
            "max_final_rel_residual": float(max(np.max(r0[good]), np.max(r1[good])))}
     log(f"[eval] {json.dumps(res)}")
@@ -227,11 +245,23 @@ def main(argv=None):
     ap.add_argument("--max-cycles", type=int, default=50000)
     ap.add_argument("--guess-sweeps", type=int, default=4)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--eval-only", default=None, help="state file of a trained network: evaluate it only")
     a = ap.parse_args(argv)
     dev = torch.device("cuda", 0)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+    if a.eval_only:
+        net = UNet(seed=a.seed, device=dev)
+        net.load_state_dict(torch.load(a.eval_only, weights_only=True))
+        out = {}
+        for judge in ("rhs", "cycle"):
+            for sw in sorted({0, a.guess_sweeps}):
+                out[f"{judge}_sweeps{sw}"] = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev, gamma=a.gamma,
+                                                      log=log, max_cycles=a.max_cycles, guess_sweeps=sw,
+                                                      judge=judge)
+        print(json.dumps(out), flush=True)
+        return
     t0 = time.perf_counter()
     X, Y, counts = generate(a.samples, a.m, a.gen_batch, a.seed, a.label_rtol, dev, log, a.gamma, a.max_cycles)
     t_data = time.perf_counter() - t0
