@@ -115,7 +115,11 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         thread per red/black column pair with the
  *                         coefficients loaded one plane ahead (levels with
  *                         mx <= 512, rows 2, 4, 5, 6 or 14; others use 1);
- *                         1 one thread per ring point */
+ *                         1 one thread per ring point
+ *   JSV_TUNE_FUSE_PROLONG in 3D the first post-smoothing pair sweep of a
+ *                         level applies the prolongation from the coarser
+ *                         level itself: 1 (default) FP32 levels only, 2 FP32
+ *                         and FP64, 0 never (separate prolongation pass) */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -125,6 +129,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_GRAPHS 6
 #define JSV_TUNE_COARSE_M 7
 #define JSV_TUNE_SWEEP_VARIANT 8
+#define JSV_TUNE_FUSE_PROLONG 9
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

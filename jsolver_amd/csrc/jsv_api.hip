@@ -118,6 +118,7 @@ struct jsv_plan {
     int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024)
     int sweep_variant = 2;          // 1: thread per ring point; 2: thread per column pair, pipelined
     int n_cu = 256;                 // compute units of the device
+    int fuse_prolong = 1;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
     bool use_graphs = true;         // replay whole cycles as captured hipGraphs
     int coarse_m = 0;               // levels with max resolution <= this run as one coarse program (0: off)
     void* lv_dev[2] = {nullptr, nullptr};          // LvPtr<double> / LvPtr<float> per level
@@ -220,19 +221,23 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
     return 0;
 }
 
-// LDS of the pair sweep with TY rows (rings + held black half)
-size_t pair_sweep_lds(const Geo& g, int ty, size_t w)
+// LDS of the pair sweep with TY rows (rings + held black half, plus the
+// coarse-plane ring when the prolongation from coarse level gc is fused)
+size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr)
 {
-    return w * ((size_t)(3 * (ty + 4) + 3 * (ty + 2)) * ((g.nx + 1) / 2) + 8 * 1024);
+    size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * ((g.nx + 1) / 2) + 8 * 1024;
+    if (gc != nullptr) e += (size_t)3 * (ty / 2 + 4) * gc->nx;
+    return w * e;
 }
 
-bool pair_sweep_rows_fit(const Geo& g, int ty, size_t w)
+bool pair_sweep_rows_fit(const Geo& g, int ty, size_t w, const Geo* gc = nullptr)
 {
-    return (ty + 2) * (g.mx / 2) <= 1024 && (g.nx + 1) / 2 <= 258 && pair_sweep_lds(g, ty, w) <= 160 * 1024 &&
-           (double)g.n * w < 4294967296.0;
+    return (ty + 2) * (g.mx / 2) <= 1024 && (g.nx + 1) / 2 <= 258 &&
+           pair_sweep_lds(g, ty, w, gc) <= 160 * 1024 && (double)g.n * w < 4294967296.0 &&
+           (gc == nullptr || (double)gc->n * w < 4294967296.0);
 }
 
-bool pair_sweep_fits(const Geo& g, size_t w) { return pair_sweep_rows_fit(g, 2, w); }
+bool pair_sweep_fits(const Geo& g, size_t w, const Geo* gc = nullptr) { return pair_sweep_rows_fit(g, 2, w, gc); }
 
 template <typename T, int TY, int NHM>
 int pair_sweep_per_cu(const Geo& g)
@@ -264,27 +269,32 @@ int pair_sweep_per_cu(const Geo& g, int ty)
 }
 
 template <typename T, int TY, int NHM>
-int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, hipStream_t s)
+int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s)
 {
     const Geo& g = H.g[l];
     constexpr int NT = 1024;
     const int nh = (g.nx + 1) / 2;
-    if ((TY + 2) * (g.mx / 2) > NT || nh > NHM || (double)g.n * sizeof(T) >= 4294967296.0)
+    const Geo gc = pc != nullptr ? H.g[l + 1] : g;
+    if ((TY + 2) * (g.mx / 2) > NT || nh > NHM || !pair_sweep_rows_fit(g, TY, sizeof(T), pc ? &gc : nullptr))
         return fail("pair sweep does not fit level %d", l);
-    const size_t lds = pair_sweep_lds(g, TY, sizeof(T));
-    if (lds > 160 * 1024) return fail("pair sweep tile needs %zu bytes of LDS", lds);
+    const size_t lds = pair_sweep_lds(g, TY, sizeof(T), pc ? &gc : nullptr);
     const int ytiles = (g.my - 1 + TY - 1) / TY;
     const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
-    hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM>), dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds, s,
-                       H.phi[l], H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles);
+    const dim3 grid((unsigned)(ytiles * zchunks));
+    if (pc != nullptr)
+        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, true>), grid, dim3(NT), lds, s, H.phi[l], H.phi2[l],
+                           H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pc, gc);
+    else
+        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, false>), grid, dim3(NT), lds, s, H.phi[l], H.phi2[l],
+                           H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, (const T*)nullptr, gc);
     return 0;
 }
 
 template <typename T, int TY>
-int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, hipStream_t s)
+int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s)
 {
-    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130>(P, H, l, zchunk, s);
-    return launch_rbsweep3p_t<T, TY, 258>(P, H, l, zchunk, s);
+    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130>(P, H, l, zchunk, pc, s);
+    return launch_rbsweep3p_t<T, TY, 258>(P, H, l, zchunk, pc, s);
 }
 
 // Rows and z-chunk of the pair sweep.  Untuned, every fitting row count is
@@ -292,18 +302,19 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, hipStre
 // the share of pair rows that are not halo, TY / (TY + 2); the z-chunk then
 // gives each slot one tile.
 template <typename T>
-int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s)
 {
     const Geo& g = H.g[l];
+    const Geo* gc = pc != nullptr ? &H.g[l + 1] : nullptr;
     int rows = P->sweep_rows;
     if (rows == 8 || rows == 16) rows = 6;
-    const bool tuned = rows > 0 && pair_sweep_rows_fit(g, rows, sizeof(T));
+    const bool tuned = rows > 0 && pair_sweep_rows_fit(g, rows, sizeof(T), gc);
     int zchunks = 0;
     if (!tuned) {
         rows = 0;
         double best = -1.0;
         for (int ty : {14, 6, 5, 4, 2}) {
-            if (!pair_sweep_rows_fit(g, ty, sizeof(T))) continue;
+            if (!pair_sweep_rows_fit(g, ty, sizeof(T), gc)) continue;
             const int slots = P->n_cu * pair_sweep_per_cu<T>(g, ty);
             const int ytiles = (g.my - 1 + ty - 1) / ty;
             int zc = slots / ytiles;
@@ -330,19 +341,20 @@ int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
         zchunk = (g.mz - 1 + zchunks - 1) / zchunks;
     }
     switch (rows) {
-    case 14: return launch_rbsweep3p_t<T, 14>(P, H, l, zchunk, s);
-    case 6: return launch_rbsweep3p_t<T, 6>(P, H, l, zchunk, s);
-    case 5: return launch_rbsweep3p_t<T, 5>(P, H, l, zchunk, s);
-    case 4: return launch_rbsweep3p_t<T, 4>(P, H, l, zchunk, s);
-    default: return launch_rbsweep3p_t<T, 2>(P, H, l, zchunk, s);
+    case 14: return launch_rbsweep3p_t<T, 14>(P, H, l, zchunk, pc, s);
+    case 6: return launch_rbsweep3p_t<T, 6>(P, H, l, zchunk, pc, s);
+    case 5: return launch_rbsweep3p_t<T, 5>(P, H, l, zchunk, pc, s);
+    case 4: return launch_rbsweep3p_t<T, 4>(P, H, l, zchunk, pc, s);
+    default: return launch_rbsweep3p_t<T, 2>(P, H, l, zchunk, pc, s);
     }
 }
 
 // tile rows x threads per workgroup of the fused sweep (tunable; default by precision)
 template <typename T>
-int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr)
 {
-    if (P->sweep_variant == 2 && pair_sweep_fits(H.g[l], sizeof(T))) return launch_rbsweep3p<T>(P, H, l, s);
+    if (pc != nullptr) return launch_rbsweep3p<T>(P, H, l, pc, s);
+    if (P->sweep_variant == 2 && pair_sweep_fits(H.g[l], sizeof(T))) return launch_rbsweep3p<T>(P, H, l, nullptr, s);
     int rows = P->sweep_rows > 0 ? P->sweep_rows : (sizeof(T) == 4 ? 16 : 4);
     const int nt = P->sweep_threads > 0 ? P->sweep_threads : 1024;
     if (nt == 256) {
@@ -357,8 +369,24 @@ int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
     return launch_rbsweep3_t<T, 16, 1024>(P, H, l, s);
 }
 
+// The 3D post-smoothing sweep of level l can apply the prolongation from
+// level l + 1 itself (pair sweep with the coarse-plane ring).
 template <typename T>
-int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+bool prolong_fusable(const jsv_plan* P, const Hier<T>& H, int l)
+{
+    // FP64 is off by default: the coarse-ring code makes the FP64 kernel
+    // spill registers and it loses more than the separate pass costs
+    if (P->dim != 3 || P->fuse_prolong == 0 || (sizeof(T) == 8 && P->fuse_prolong < 2) || l + 1 >= P->levels)
+        return false;
+    const Geo& g = H.g[l];
+    if (!(P->fused_min_m > 0 && g.mz >= P->fused_min_m && H.phi2[l] != nullptr)) return false;
+    return P->sweep_variant == 2 && pair_sweep_fits(g, sizeof(T), &H.g[l + 1]);
+}
+
+// pc: coarse correction to prolongate into level l first (3D fused pair sweep
+// only; see prolong_fusable)
+template <typename T>
+int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr)
 {
     const Geo& g = H.g[l];
     if (P->dim == 2) {
@@ -383,7 +411,7 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
         }
     } else if (P->fused_min_m > 0 && g.mz >= P->fused_min_m && H.phi2[l] != nullptr) {
         // fused out-of-place sweep, then swap the level's phi buffers
-        RETCHK(launch_rbsweep3<T>(P, H, l, s));
+        RETCHK(launch_rbsweep3<T>(P, H, l, s, pc));
         std::swap(H.phi[l], H.phi2[l]);
     } else {
         // Slab schedule: red on z-slab k, then black on slab k-1, so the black
@@ -615,6 +643,12 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
         if (fcyc)
             for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, true, false, s));
         for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, false, false, s));
+    }
+    if (c.nu2 >= 1 && prolong_fusable(P, H, l)) {
+        // the first post-smoothing sweep applies the prolongation itself
+        RETCHK(smooth(P, H, l, s, (const T*)H.phi[l + 1]));
+        for (int i = 1; i < c.nu2; ++i) RETCHK(smooth(P, H, l, s));
+        return 0;
     }
     RETCHK(prolong(P, H, l, c.nu2, s));
     for (int i = 0; i < c.nu2; ++i) RETCHK(smooth(P, H, l, s));
@@ -940,6 +974,11 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     }
     if (key == JSV_TUNE_ZCHUNK) {
         P->zchunk = (int)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_FUSE_PROLONG) {
+        if (value > 2) return fail("fuse prolong must be 0, 1 or 2");
+        P->fuse_prolong = (int)value;
         return 0;
     }
     if (key == JSV_TUNE_SWEEP_VARIANT) {

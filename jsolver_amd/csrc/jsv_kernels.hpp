@@ -140,6 +140,86 @@ k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
     rb3_point(phi, rhs, cx, cy, cz, cc, (long)z * g.sz + (long)y * g.sy + x, g.sy, g.sz);
 }
 
+// Prolongation corrections (solver_2d.py:649-668, solver_3d.py:654-677), shared by
+// the prolongation kernels and the pair sweep that applies the correction on the fly.
+// Correction at fine point (x, y[, z]) from coarse array pc (problem offset
+// included); false where the point is skipped (full == 0).
+template <typename T>
+__device__ __forceinline__ bool prolong2_corr(const T* __restrict__ pc, long csy, int x, int y,
+                                              int full, T& corr)
+{
+    const int ox = x & 1, oy = y & 1;
+    const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Xl = x >> 1, Yl = y >> 1;
+#define C2(J, K) pc[(long)(J) * csy + (K)]
+    if (ox && oy) {
+        corr = C2(Yo, Xo);
+    } else if (oy) {
+        corr = (T)0.5 * (C2(Yo, Xl + 1) + C2(Yo, Xl));
+    } else if (ox) {
+        corr = (T)0.5 * (C2(Yl + 1, Xo) + C2(Yl, Xo));
+    } else {
+        if (!full) return false;
+        T s = C2(Yl + 1, Xl + 1) + C2(Yl, Xl + 1);
+        s = s + C2(Yl + 1, Xl);
+        s = s + C2(Yl, Xl);
+        corr = (T)0.25 * s;
+    }
+#undef C2
+    return true;
+}
+
+// C3(I, J, K) reads the coarse correction at coarse point (I, J, K)
+template <typename T, typename Coarse>
+__device__ __forceinline__ bool prolong3_corr_at(Coarse C3, int x, int y, int z, int full, T& corr)
+{
+    const int ox = x & 1, oy = y & 1, oz = z & 1;
+    const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Zo = (z + 1) >> 1;
+    const int Xl = x >> 1, Yl = y >> 1, Zl = z >> 1;
+    const int nodd = ox + oy + oz;
+    if (nodd == 3) {
+        corr = C3(Zo, Yo, Xo);
+    } else if (nodd == 2) {
+        if (!ox)      corr = (T)0.5 * (C3(Zo, Yo, Xl + 1) + C3(Zo, Yo, Xl));
+        else if (!oy) corr = (T)0.5 * (C3(Zo, Yl + 1, Xo) + C3(Zo, Yl, Xo));
+        else          corr = (T)0.5 * (C3(Zl + 1, Yo, Xo) + C3(Zl, Yo, Xo));
+    } else if (nodd == 1) {
+        if (!full) return false;
+        T s;
+        if (oz) {
+            s = C3(Zo, Yl + 1, Xl + 1) + C3(Zo, Yl, Xl + 1);
+            s = s + C3(Zo, Yl + 1, Xl);
+            s = s + C3(Zo, Yl, Xl);
+        } else if (oy) {
+            s = C3(Zl + 1, Yo, Xl + 1) + C3(Zl, Yo, Xl + 1);
+            s = s + C3(Zl + 1, Yo, Xl);
+            s = s + C3(Zl, Yo, Xl);
+        } else {
+            s = C3(Zl + 1, Yl + 1, Xo) + C3(Zl, Yl + 1, Xo);
+            s = s + C3(Zl + 1, Yl, Xo);
+            s = s + C3(Zl, Yl, Xo);
+        }
+        corr = (T)0.25 * s;
+    } else {
+        T s = C3(Zl + 1, Yl + 1, Xl + 1) + C3(Zl, Yl + 1, Xl + 1);
+        s = s + C3(Zl + 1, Yl, Xl + 1);
+        s = s + C3(Zl, Yl, Xl + 1);
+        s = s + C3(Zl + 1, Yl + 1, Xl);
+        s = s + C3(Zl, Yl + 1, Xl);
+        s = s + C3(Zl + 1, Yl, Xl);
+        s = s + C3(Zl, Yl, Xl);
+        corr = (T)0.125 * s;
+    }
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ bool prolong3_corr(const T* __restrict__ pc, long csy, long csz, int x,
+                                              int y, int z, int full, T& corr)
+{
+    return prolong3_corr_at<T>([&](int I, int J, int K) -> T { return pc[(long)I * csz + (long)J * csy + K]; },
+                               x, y, z, full, corr);
+}
+
 // ---------------------------------------------------------------------------
 // Fused RBGS sweep (red half then black half), out of place, marching in z.
 //
@@ -305,7 +385,9 @@ k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict_
 // a register.  The two coefficient sets alternate between two register
 // blocks (the loop is unrolled by two), so the loads issued for plane zs+1
 // stay in flight across the step's barriers and compute; the black half
-// waits in LDS.  Requires (TY + 2) * (mx / 2) <= NT, (nx + 1) / 2 <= NHMAX
+// waits in LDS.  With PRO the prolongation that precedes a post-smoothing
+// sweep is applied on the fly to the black phi_in values (the only ones the
+// sweep reads).  Requires (TY + 2) * (mx / 2) <= NT, (nx + 1) / 2 <= NHMAX
 // and the level below 4 GiB; rim values of phi_in are zero.
 // ---------------------------------------------------------------------------
 // two consecutive elements, loaded as one dword-aligned vector access
@@ -322,20 +404,23 @@ struct PairCo {
     T rl, rr, ccl, ccr;       // rhs, diagonal at 2k, 2k+1
 };
 
-template <typename T, int TY, int NT, int NHMAX, int WPE = 1>
+template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int WPE = 1>
 __global__ void __launch_bounds__(NT, WPE)
 k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-            const T* __restrict__ cc, Geo g, int zchunk, int ytiles)
+            const T* __restrict__ cc, Geo g, int zchunk, int ytiles, const T* __restrict__ pc, Geo gc)
 {
     constexpr int PR = TY + 4;                  // rows of the phi_in ring: y0-2 .. y0+TY+1
     constexpr int RR = TY + 2;                  // rows of the red ring / pair rows: y0-1 .. y0+TY
     constexpr int PMAX = (PR * NHMAX + NT - 1) / NT;   // ring elements per thread (nh <= NHMAX)
+    constexpr int CRR = TY / 2 + 4;             // coarse rows under the phi_in ring rows
+    constexpr int CMAX = (CRR * (NHMAX + 1) + NT - 1) / NT;   // coarse ring elements per thread
     const int nh = (g.nx + 1) >> 1;
     const int nhe = g.mx >> 1;                  // pairs per row
     T* P = reinterpret_cast<T*>(jsv_dyn_smem);  // [3][PR][nh]
     T* R = P + 3 * PR * nh;                     // [3][RR][nh]
     T* Hb = R + 3 * RR * nh;                    // [8][NT] this thread's held black half
+    T* Cr = Hb + 8 * NT;                        // PRO: [3][CRR][gc.nx] coarse planes of the correction
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int y0 = 2 + (tile % ytiles) * TY;
@@ -375,12 +460,12 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         ok = i < PR * nh && z >= 0 && z < g.nz && yy >= 0 && yy < g.ny && x < g.nx;
         return ok ? (unsigned)(((long)z * sz + (long)yy * sy + x) * (long)sizeof(T)) : 0u;
     };
-    auto load_plane = [&](int z) {
+    auto load_plane = [&](int z, auto corr_fn) {
         T* dst = P + (z % 3) * PR * nh;
         for (int i = tid; i < PR * nh; i += NT) {
             bool ok;
             const unsigned o = src(z, i, ok);
-            dst[i] = ok ? at(pin, o) : (T)0;
+            dst[i] = corr_fn(z, i, ok ? at(pin, o) : (T)0);
         }
     };
     T pre[PMAX];
@@ -395,12 +480,97 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
             pre_ok |= ok ? (1u << j) : 0u;
         }
     };
+    // PRO: the prolongation phi += P e (solver_3d.py:654-677) is applied to
+    // phi_in's black values as they enter the ring (with nu2 > 0 the red
+    // points' corrections are never read).  e comes from a ring of three
+    // coarse planes; fine plane f needs coarse planes (f+1)>>1 .. (f+2)>>1.
+    const int cnx = gc.nx;
+    const int cr0 = (y0 - 2) >> 1;              // first coarse row of the ring
+    auto cval = [&](int I, int J, int K) -> T { return Cr[((I % 3) * CRR + (J - cr0)) * cnx + K]; };
+    // correction of ring element i of fine plane z (false: none)
+    auto corr_of = [&](int z, int i, T& corr) -> bool {
+        const int rr_ = i / nh, kk = i - rr_ * nh;
+        const int yy = y0 - 2 + rr_;
+        const int x = 2 * kk + ((yy + z) & 1);
+        return x >= 2 && x <= g.mx && yy >= 2 && yy <= g.my && z >= 2 && z <= g.mz &&
+               prolong3_corr_at<T>(cval, x, yy, z, 0, corr);
+    };
+    auto corrected = [&](int z, int i, T v) -> T {
+        if constexpr (PRO) {
+            T corr;
+            if (corr_of(z, i, corr)) v = v + corr;
+        }
+        return v;
+    };
+    // The corrections of this thread's ring elements of plane z are computed
+    // at the end of the step before its commit and parked in the element
+    // itself (that ring slot is free by then); the commit adds the loaded
+    // value.  Nothing stays live in registers across the step boundary.
+    unsigned cpr_ok = 0;
+    auto prep_corr = [&](int z) {
+        if constexpr (PRO) {
+            T* dst = P + (z % 3) * PR * nh;
+            cpr_ok = 0;
+#pragma unroll
+            for (int j = 0; j < PMAX; ++j) {
+                const int i = tid + j * NT;
+                T c;
+                if (i < PR * nh && corr_of(z, i, c)) {
+                    dst[i] = c;
+                    cpr_ok |= 1u << j;
+                }
+            }
+        }
+    };
+    const T* pcb = PRO ? pc : pin;              // a valid base for the unconditional loads
+    auto csrc = [&](int I, int i, bool& ok) -> unsigned {
+        const int rr_ = i / cnx, kk = i - rr_ * cnx;
+        const int J = cr0 + rr_;
+        ok = i < CRR * cnx && J < gc.ny && I >= 0 && I < gc.nz;
+        return ok ? (unsigned)(((long)I * gc.sz + (long)J * gc.sy + kk) * (long)sizeof(T)) : 0u;
+    };
+    auto cload_plane = [&](int I) {
+        T* dst = Cr + (I % 3) * CRR * cnx;
+        for (int i = tid; i < CRR * cnx; i += NT) {
+            bool ok;
+            const unsigned o = csrc(I, i, ok);
+            dst[i] = ok ? at(pcb, o) : (T)0;
+        }
+    };
+    T cpre[CMAX];
+    unsigned cpre_ok = 0;
+    auto cissue = [&](int I) {
+        cpre_ok = 0;
+#pragma unroll
+        for (int j = 0; j < CMAX; ++j) {
+            bool ok;
+            const unsigned o = csrc(I, tid + j * NT, ok);
+            cpre[j] = at(pcb, o);
+            cpre_ok |= ok ? (1u << j) : 0u;
+        }
+    };
+    auto ccommit = [&](int I) {
+        T* dst = Cr + (I % 3) * CRR * cnx;
+#pragma unroll
+        for (int j = 0; j < CMAX; ++j) {
+            const int i = tid + j * NT;
+            if (i < CRR * cnx) dst[i] = ((cpre_ok >> j) & 1u) ? cpre[j] : (T)0;
+        }
+    };
+    auto cmax = [](int f) { return (f + 2) >> 1; };
+
     auto commit = [&](int z) {
         T* dst = P + (z % 3) * PR * nh;
 #pragma unroll
         for (int j = 0; j < PMAX; ++j) {
             const int i = tid + j * NT;
-            if (i < PR * nh) dst[i] = ((pre_ok >> j) & 1u) ? pre[j] : (T)0;
+            if (i < PR * nh) {
+                T v = ((pre_ok >> j) & 1u) ? pre[j] : (T)0;
+                if constexpr (PRO) {
+                    if ((cpr_ok >> j) & 1u) v = v + dst[i];
+                }
+                dst[i] = v;
+            }
         }
     };
     auto Pv = [&](int z, int yy, int x) -> T {
@@ -435,9 +605,16 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         c.ccr = v.b;
     };
 
-    load_plane(z0 - 2);
-    load_plane(z0 - 1);
+    int chave = 0;                              // PRO: highest coarse plane in the ring
+    if constexpr (PRO) {
+        chave = cmax(z0 + 1);
+        for (int I = (z0 - 1) >> 1; I <= chave; ++I) cload_plane(I);
+        __syncthreads();
+    }
+    load_plane(z0 - 2, corrected);
+    load_plane(z0 - 1, corrected);
     issue(z0);
+    prep_corr(z0);
     PairCo<T> A, B;
     load(z0 - 1, A);
     T czpl, czpr;                               // cz of the plane below the current one
@@ -455,6 +632,16 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         // unconditional issue: planes up to z1 + 2 <= nz are requested (the
         // ring's guard zeroes plane nz; coefficient plane zs + 1 <= nz - 1)
         commit(zs + 1);
+        if constexpr (PRO) {
+            // coarse plane for the next step's commit (issued one step ago);
+            // it replaces a plane no commit of this step reads
+            const int want = cmax(zs + 2);
+            if (want > chave) {
+                ccommit(want);
+                chave = want;
+            }
+            cissue(cmax(zs + 3));
+        }
         issue(zs + 2);
         load(zs + 1, nxt);
         __syncthreads();
@@ -513,6 +700,10 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         red_prev = red;
         czpl = cur.czl;
         czpr = cur.czr;
+        // coarse planes up to cmax(zs + 2) are in the ring since this step's
+        // first barrier; ring slot (zs + 2) % 3 last held plane zs - 1, read
+        // by this step's red half before the second barrier
+        prep_corr(zs + 2);
     };
     for (int zs = z0 - 1; zs <= z1; zs += 2) {
         step(zs, A, B);
@@ -721,78 +912,6 @@ k_restrict_gather2(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc,
 // i/2 and i/2+1.  The interpolations the reference skips when nu2 > 0 (points
 // of the first RBGS half, overwritten before they are read) are skipped alike.
 // ---------------------------------------------------------------------------
-// Correction at fine point (x, y[, z]) from coarse array pc (problem offset
-// included); false where the point is skipped (full == 0).
-template <typename T>
-__device__ __forceinline__ bool prolong2_corr(const T* __restrict__ pc, long csy, int x, int y,
-                                              int full, T& corr)
-{
-    const int ox = x & 1, oy = y & 1;
-    const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Xl = x >> 1, Yl = y >> 1;
-#define C2(J, K) pc[(long)(J) * csy + (K)]
-    if (ox && oy) {
-        corr = C2(Yo, Xo);
-    } else if (oy) {
-        corr = (T)0.5 * (C2(Yo, Xl + 1) + C2(Yo, Xl));
-    } else if (ox) {
-        corr = (T)0.5 * (C2(Yl + 1, Xo) + C2(Yl, Xo));
-    } else {
-        if (!full) return false;
-        T s = C2(Yl + 1, Xl + 1) + C2(Yl, Xl + 1);
-        s = s + C2(Yl + 1, Xl);
-        s = s + C2(Yl, Xl);
-        corr = (T)0.25 * s;
-    }
-#undef C2
-    return true;
-}
-
-template <typename T>
-__device__ __forceinline__ bool prolong3_corr(const T* __restrict__ pc, long csy, long csz, int x,
-                                              int y, int z, int full, T& corr)
-{
-    const int ox = x & 1, oy = y & 1, oz = z & 1;
-    const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Zo = (z + 1) >> 1;
-    const int Xl = x >> 1, Yl = y >> 1, Zl = z >> 1;
-#define C3(I, J, K) pc[(long)(I) * csz + (long)(J) * csy + (K)]
-    const int nodd = ox + oy + oz;
-    if (nodd == 3) {
-        corr = C3(Zo, Yo, Xo);
-    } else if (nodd == 2) {
-        if (!ox)      corr = (T)0.5 * (C3(Zo, Yo, Xl + 1) + C3(Zo, Yo, Xl));
-        else if (!oy) corr = (T)0.5 * (C3(Zo, Yl + 1, Xo) + C3(Zo, Yl, Xo));
-        else          corr = (T)0.5 * (C3(Zl + 1, Yo, Xo) + C3(Zl, Yo, Xo));
-    } else if (nodd == 1) {
-        if (!full) return false;
-        T s;
-        if (oz) {
-            s = C3(Zo, Yl + 1, Xl + 1) + C3(Zo, Yl, Xl + 1);
-            s = s + C3(Zo, Yl + 1, Xl);
-            s = s + C3(Zo, Yl, Xl);
-        } else if (oy) {
-            s = C3(Zl + 1, Yo, Xl + 1) + C3(Zl, Yo, Xl + 1);
-            s = s + C3(Zl + 1, Yo, Xl);
-            s = s + C3(Zl, Yo, Xl);
-        } else {
-            s = C3(Zl + 1, Yl + 1, Xo) + C3(Zl, Yl + 1, Xo);
-            s = s + C3(Zl + 1, Yl, Xo);
-            s = s + C3(Zl, Yl, Xo);
-        }
-        corr = (T)0.25 * s;
-    } else {
-        T s = C3(Zl + 1, Yl + 1, Xl + 1) + C3(Zl, Yl + 1, Xl + 1);
-        s = s + C3(Zl + 1, Yl, Xl + 1);
-        s = s + C3(Zl, Yl, Xl + 1);
-        s = s + C3(Zl + 1, Yl + 1, Xl);
-        s = s + C3(Zl, Yl + 1, Xl);
-        s = s + C3(Zl + 1, Yl, Xl);
-        s = s + C3(Zl, Yl, Xl);
-        corr = (T)0.125 * s;
-    }
-#undef C3
-    return true;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_prolong2(T* __restrict__ phi, const T* __restrict__ pcb, Geo gf, Geo gc, int full,

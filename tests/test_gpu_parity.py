@@ -131,6 +131,52 @@ def test_fused_sweep_bitexact(ms, zchunk, variant, rows):
         np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle))
 
 
+@pytest.mark.parametrize("fuse", [2, 0])
+@pytest.mark.parametrize("rows", [0, 2, 5])
+@pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 16), (8, 8, 8), (32, 32, 64)])
+def test_fused_prolongation_cycles_bitexact(ms, rows, fuse):
+    """Post-smoothing pair sweeps that apply the prolongation themselves (a
+    coarse-plane ring in LDS) give the oracle's bits for whole cycles: V, W,
+    F, nu2 = 2 and 3, skipped pre-smoothing."""
+    from jsolver_amd import _lib
+    case = ("3d_fused_pro", 3, ms, True, True, False)
+    dp, op, phi0, rhs0 = _setup(case, seed=31)
+    lib = _lib.load()
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSED_MIN_M, 2))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_VARIANT, 2))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_ROWS, rows))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_PROLONG, fuse))
+    for kw in (dict(), dict(gamma=2), dict(fcycle=True), dict(nu1=1, nu2=2), dict(gamma=2, nu1=2, nu2=3),
+               dict(skip_pre=True)):
+        dp.upload(0, 0, phi0)
+        dp.upload(0, 1, rhs0)
+        dp.cycle(cfg(**kw), skip_pre=kw.get("skip_pre", False))
+        c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+        want = gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle, 0, c.skip_pre)
+        np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=str(kw))
+
+
+@pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 16)])
+def test_fused_prolongation_fp32_matches_separate(ms):
+    """FP32 cycles (the mixed solver's inner cycles): the fused prolongation
+    (default for FP32) gives the same bits as the separate prolongation pass."""
+    from jsolver_amd import _lib
+    phi, rhs, lam, D, _ = rand_problem(3, ms, seed=37)
+    outs = []
+    for fuse in (1, 0):
+        dp = DevPlan(3, ms, lam, D, precision="mixed")
+        lib = _lib.load()
+        _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSED_MIN_M, 2))
+        _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_PROLONG, fuse))
+        dp.upload(0, 0, gmg.zero_rim(phi).astype(np.float32), prec=_lib.JSV_MIXED)
+        dp.upload(0, 1, rhs.astype(np.float32), prec=_lib.JSV_MIXED)
+        for kw in (dict(), dict(gamma=2, nu2=2)):
+            dp.cycle(cfg(**kw), prec=_lib.JSV_MIXED)
+        outs.append(dp.download(0, 0, prec=_lib.JSV_MIXED))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    assert np.abs(outs[0]).max() > 0
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_restrict_two_pass_bitexact(case):
     """The two-pass restriction of large levels (residuals of the first RBGS
