@@ -111,6 +111,9 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         every coarser level) run each coarse sub-cycle as
  *                         one single-workgroup launch per problem
  *                         (default 16 in 3D, 64 in 2D; 0 = off)
+ *   JSV_TUNE_COARSE_LDS   1: that launch keeps its coarsest levels (up to
+ *                         150 KB) in LDS; 0: all levels in HBM (default 1 in
+ *                         3D, 0 in 2D)
  *   JSV_TUNE_SWEEP_VARIANT kernel of the fused sweep: 2 (default, 0) one
  *                         thread per red/black column pair with the
  *                         coefficients loaded one plane ahead (levels with
@@ -130,6 +133,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_COARSE_M 7
 #define JSV_TUNE_SWEEP_VARIANT 8
 #define JSV_TUNE_FUSE_PROLONG 9
+#define JSV_TUNE_COARSE_LDS 10
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

@@ -5,6 +5,7 @@
 // that enqueues kernels on the caller's stream.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -123,6 +124,9 @@ struct jsv_plan {
     int coarse_m = 0;               // levels with max resolution <= this run as one coarse program (0: off)
     void* lv_dev[2] = {nullptr, nullptr};          // LvPtr<double> / LvPtr<float> per level
     std::vector<const void*> lv_host[2];           // pointers lv_dev holds (start level + table)
+    int coarse_lds = 1;             // stage the program's coarsest levels in LDS (set for 3D at creation)
+    int clds_from[2] = {0, 0}, clds_to[2] = {0, 0};   // staged level range per precision
+    size_t clds_bytes[2] = {0, 0};
     struct CoarseProg {
         int prec, gamma, nu1, nu2, fcyc, start, offdiag;
         jsv::CoarseOp* ops;
@@ -588,14 +592,43 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
     // stale values) after the plan switches back to a 5-point operator
     auto drr = [&](int l) -> T* { return P->offdiag ? H.drr[l] : nullptr; };
     auto dlr = [&](int l) -> T* { return P->offdiag ? H.dlr[l] : nullptr; };
-    std::vector<const void*> key{(const void*)(long)start};
+    // LDS staging: the coarsest levels (not level L-1, which no cycle
+    // visits) from the bottom up while they fit 150 KB
+    int lds_from = P->levels - 1, lds_to = P->levels - 1;
+    size_t used = 0;
+    std::vector<std::array<int, 8>> loff(P->levels);
+    for (auto& a : loff) a.fill(-1);
+    if (P->coarse_lds) {
+        for (int l = P->levels - 2; l >= start; --l) {
+            const Geo& g = H.g[l];
+            const size_t np = P->dim == 3 ? (size_t)g.n : (size_t)g.sz;
+            const bool has[8] = {true, true, true, true, P->dim == 3, true, P->offdiag, P->offdiag};
+            size_t nar = 0;
+            for (bool h : has) nar += h ? 1 : 0;
+            if (used + np * nar * sizeof(T) > 150 * 1024) break;
+            size_t o = used / sizeof(T);
+            for (int j = 0; j < 8; ++j)
+                if (has[j]) {
+                    loff[l][j] = (int)o;
+                    o += np;
+                }
+            used += np * nar * sizeof(T);
+            lds_from = l;
+        }
+    }
+    P->clds_from[pi] = lds_from;
+    P->clds_to[pi] = lds_to;
+    P->clds_bytes[pi] = used;
+    std::vector<const void*> key{(const void*)(long)start, (const void*)(long)lds_from};
     for (int l = start; l < P->levels; ++l)
         for (const T* q : {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l)})
             key.push_back(q);
     if (P->lv_host[pi] != key) {
         std::vector<jsv::LvPtr<T>> lv(P->levels);
-        for (int l = start; l < P->levels; ++l)
-            lv[l] = {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l), H.g[l]};
+        for (int l = start; l < P->levels; ++l) {
+            lv[l] = {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l), H.g[l], {}};
+            for (int j = 0; j < 8; ++j) lv[l].lds[j] = loff[l][j];
+        }
         const size_t nb = sizeof(jsv::LvPtr<T>) * lv.size();
         if (P->lv_dev[pi] == nullptr) RETCHK(dalloc(P, &P->lv_dev[pi], nb));
         HIPCHK(hipStreamSynchronize(s));   // a queued cycle may still read the table
@@ -635,9 +668,16 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
     if (pr != nullptr) {
         // zero + the gamma (F: 2 gamma) sub-cycles of level l + 1 in one launch
         const int pi = sizeof(T) == 4 ? 1 : 0;
-        hipLaunchKernelGGL(jsv::k_coarse_prog<T>, dim3((unsigned)(P->dim == 2 ? P->batch : 1)),
-                           dim3(jsv::COARSE_THREADS), 0, s, (const jsv::LvPtr<T>*)P->lv_dev[pi],
-                           pr->ops, pr->nops, P->dim, P->act_ptr());
+        const int lf = P->clds_from[pi], lt = P->clds_to[pi];
+        const dim3 grid((unsigned)(P->dim == 2 ? P->batch : 1));
+        const auto* lvp = (const jsv::LvPtr<T>*)P->lv_dev[pi];
+        if (lf < lt)
+            hipLaunchKernelGGL((jsv::k_coarse_prog<T, true>), grid, dim3(jsv::COARSE_THREADS), P->clds_bytes[pi], s,
+                               lvp, pr->ops, pr->nops, P->dim, P->act_ptr(), lf, lt,
+                               (lf <= start && start < lt) ? start : -1);
+        else
+            hipLaunchKernelGGL((jsv::k_coarse_prog<T, false>), grid, dim3(jsv::COARSE_THREADS), 0, s, lvp, pr->ops,
+                               pr->nops, P->dim, P->act_ptr(), lf, lt, -1);
     } else {
         RETCHK(zero(H.phi[l + 1], H.g[l + 1].n, s));
         if (fcyc)
@@ -847,6 +887,7 @@ int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double i
     P->prec = precision;
     P->levels = L;
     P->coarse_m = dim == 3 ? 16 : 64;
+    P->coarse_lds = dim == 3 ? 1 : 0;   // measured: helps 3D FP32, slows the 2D batched W-cycles
     {
         int dev = 0, ncu = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -970,6 +1011,12 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     }
     if (key == JSV_TUNE_COARSE_M) {
         P->coarse_m = (int)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_COARSE_LDS) {
+        P->coarse_lds = value != 0 ? 1 : 0;
+        P->lv_host[0].clear();      // restage on the next cycle
+        P->lv_host[1].clear();
         return 0;
     }
     if (key == JSV_TUNE_ZCHUNK) {

@@ -963,6 +963,10 @@ struct LvPtr {
     const T* drr;
     const T* dlr;
     Geo g;
+    // element offsets of this level's arrays in the program's LDS copy
+    // (phi, rhs, cx, cy, cz, cc, drr, dlr; -1: array absent); used when the
+    // level lies in the staged range of the launch
+    int lds[8];
 };
 
 enum CoarseOpKind : int {
@@ -979,19 +983,55 @@ struct CoarseOp {
 
 constexpr int COARSE_THREADS = 1024;
 
-template <typename T>
+// Levels [lds_from, lds_to) are staged in LDS for the whole program (this
+// problem's coefficients and rhs copied in at the start); level wb_level's
+// phi is copied back at the end (-1: none).  Staged arrays are addressed
+// through generic pointers, so the same point functions serve both kinds.
+template <typename T, bool STAGED>
 __global__ void __launch_bounds__(COARSE_THREADS)
 k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops, int nops, int dim,
-              const int* __restrict__ act)
+              const int* __restrict__ act, int lds_from, int lds_to, int wb_level)
 {
     const int b = blockIdx.x;               // 2D: problem of the batch; 3D: 0
     if (act != nullptr && act[b] == 0) return;
     const int tid = threadIdx.x;
+    T* sm = reinterpret_cast<T*>(jsv_dyn_smem);
+    // without staging every pointer stays a global one (no generic accesses)
+    auto staged = [&](int l) { return STAGED && l >= lds_from && l < lds_to; };
+    auto view = [&](int l) -> LvPtr<T> {
+        LvPtr<T> L = lv[l];
+        if (staged(l)) {
+            T* a[8];
+            for (int j = 0; j < 8; ++j) a[j] = L.lds[j] >= 0 ? sm + L.lds[j] : nullptr;
+            L.phi = a[0];
+            L.rhs = a[1];
+            L.cx = a[2];
+            L.cy = a[3];
+            L.cz = a[4];
+            L.cc = a[5];
+            L.drr = a[6];
+            L.dlr = a[7];
+        }
+        return L;
+    };
+    // problem offset of a level's arrays: none in LDS or in 3D
+    auto off = [&](int l, const Geo& g) -> long { return (dim == 2 && !staged(l)) ? (long)b * g.sz : 0; };
+    for (int l = lds_from; STAGED && l < lds_to; ++l) {
+        const LvPtr<T> G = lv[l];
+        const long np = dim == 3 ? G.g.n : G.g.sz, gb = dim == 2 ? (long)b * G.g.sz : 0;
+        const T* src[8] = {nullptr, G.rhs, G.cx, G.cy, G.cz, G.cc, G.drr, G.dlr};
+        for (int j = 1; j < 8; ++j) {
+            if (G.lds[j] < 0 || src[j] == nullptr) continue;
+            T* d = sm + G.lds[j];
+            for (long i = tid; i < np; i += COARSE_THREADS) d[i] = src[j][gb + i];
+        }
+    }
+    if (STAGED) __syncthreads();
     for (int o = 0; o < nops; ++o) {
         const CoarseOp op = ops[o];
-        const LvPtr<T> L = lv[op.level];
+        const LvPtr<T> L = view(op.level);
         const Geo& g = L.g;
-        const long base = dim == 2 ? (long)b * g.sz : 0;
+        const long base = off(op.level, g);
         if (op.kind == CO_RB) {
             const int xh = g.mx / 2, ny = g.my - 1;
             if (dim == 3) {
@@ -1023,7 +1063,7 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops,
                               base + (long)y * g.sy + x, g.sy);
             }
         } else if (op.kind == CO_RESTRICT) {
-            const LvPtr<T> C = lv[op.level + 1];
+            const LvPtr<T> C = view(op.level + 1);
             const Geo& gc = C.g;
             const int nx = gc.mx - 1, ny = gc.my - 1;
             if (dim == 3) {
@@ -1036,7 +1076,7 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops,
                         restrict3_at(L.phi, L.rhs, L.cx, L.cy, L.cz, L.cc, c, g.sy, g.sz);
                 }
             } else {
-                const long cbase = (long)b * gc.sz;
+                const long cbase = off(op.level + 1, gc);
                 for (int i = tid; i < nx * ny; i += COARSE_THREADS) {
                     const int J = 2 + i / nx, K = 2 + i % nx;
                     const long c = base + (long)(2 * J - 1) * g.sy + (2 * K - 1);
@@ -1048,7 +1088,7 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops,
             const long cnt = dim == 2 ? g.sz : g.n;
             for (long i = tid; i < cnt; i += COARSE_THREADS) L.phi[base + i] = (T)0;
         } else {  // CO_PROLONG
-            const LvPtr<T> C = lv[op.level + 1];
+            const LvPtr<T> C = view(op.level + 1);
             const Geo& gc = C.g;
             const int nx = g.mx - 1, ny = g.my - 1;
             if (dim == 3) {
@@ -1062,7 +1102,7 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops,
                     L.phi[p] = L.phi[p] + corr;
                 }
             } else {
-                const long cbase = (long)b * gc.sz;
+                const long cbase = off(op.level + 1, gc);
                 for (int i = tid; i < nx * ny; i += COARSE_THREADS) {
                     const int y = 2 + i / nx, x = 2 + i % nx;
                     T corr;
@@ -1073,6 +1113,13 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops,
             }
         }
         __syncthreads();
+    }
+    if (STAGED && wb_level >= 0 && staged(wb_level)) {
+        // the staged top level's phi goes back for the prolongation after the program
+        const LvPtr<T> G = lv[wb_level];
+        const long np = dim == 3 ? G.g.n : G.g.sz, gb = dim == 2 ? (long)b * G.g.sz : 0;
+        const T* d = sm + G.lds[0];
+        for (long i = tid; i < np; i += COARSE_THREADS) G.phi[gb + i] = d[i];
     }
 }
 

@@ -250,15 +250,18 @@ COARSE_CASES = [c for c in CASES if c[0] in ("2d_spatial_16x32", "2d_dxy_16", "3
 ]
 
 
+@pytest.mark.parametrize("lds", [1, 0])
 @pytest.mark.parametrize("coarse_m", [0, 4, 1 << 20])
 @pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4]], ids=lambda c: c[0])
 @pytest.mark.parametrize("case", COARSE_CASES, ids=[c[0] for c in COARSE_CASES])
-def test_cycle_coarse_program_bitexact(case, cyc, coarse_m):
+def test_cycle_coarse_program_bitexact(case, cyc, coarse_m, lds):
     """The single-workgroup coarse program (off, from the 4-point levels, and
-    from level 1 down) gives the oracle's bits for whole cycles."""
+    from level 1 down; its coarsest levels staged in LDS or not) gives the
+    oracle's bits for whole cycles."""
     from jsolver_amd import _lib
     dp, op, phi, rhs = _setup(case, seed=17)
     _lib.check(_lib.load().jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, coarse_m))
+    _lib.check(_lib.load().jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_LDS, lds))
     kw = cyc[1]
     c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
     for rep in range(2):                      # second call replays the cached program
