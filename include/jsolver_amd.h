@@ -119,6 +119,9 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         coefficients loaded one plane ahead (levels with
  *                         mx <= 512, rows 2, 4, 5, 6 or 14; others use 1);
  *                         1 one thread per ring point
+ *   JSV_TUNE_SWEEP_PAIRS  column pairs per thread of the pair kernel: 0
+ *                         (default) 2 for FP32 sweeps of levels with mx >= 256
+ *                         without the fused prolongation, else 1; 1; 2
  *   JSV_TUNE_FUSE_PROLONG in 3D the first post-smoothing pair sweep of a
  *                         level applies the prolongation from the coarser
  *                         level itself: 1 (default) FP32 levels only, 2 FP32
@@ -134,6 +137,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_SWEEP_VARIANT 8
 #define JSV_TUNE_FUSE_PROLONG 9
 #define JSV_TUNE_COARSE_LDS 10
+#define JSV_TUNE_SWEEP_PAIRS 11
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

@@ -118,6 +118,7 @@ struct jsv_plan {
     int sweep_rows = 0;             // rows per workgroup tile of the fused sweep (0: by precision)
     int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024)
     int sweep_variant = 2;          // 1: thread per ring point; 2: thread per column pair, pipelined
+    int sweep_pairs = 0;            // pair kernel: column pairs per thread (0: 2 in FP32, 1 in FP64)
     int n_cu = 256;                 // compute units of the device
     int fuse_prolong = 1;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
     bool use_graphs = true;         // replay whole cycles as captured hipGraphs
@@ -225,101 +226,123 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
     return 0;
 }
 
-// LDS of the pair sweep with TY rows (rings + held black half, plus the
-// coarse-plane ring when the prolongation from coarse level gc is fused)
-size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr)
+// LDS of the pair sweep with TY rows and np pairs per thread (rings + held
+// black halves, plus the coarse-plane ring when the prolongation from
+// coarse level gc is fused)
+size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1)
 {
-    size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * ((g.nx + 1) / 2) + 8 * 1024;
+    size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * ((g.nx + 1) / 2) + (size_t)8 * np * 1024;
     if (gc != nullptr) e += (size_t)3 * (ty / 2 + 4) * gc->nx;
     return w * e;
 }
 
-bool pair_sweep_rows_fit(const Geo& g, int ty, size_t w, const Geo* gc = nullptr)
+bool pair_sweep_rows_fit(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1)
 {
-    return (ty + 2) * (g.mx / 2) <= 1024 && (g.nx + 1) / 2 <= 258 &&
-           pair_sweep_lds(g, ty, w, gc) <= 160 * 1024 && (double)g.n * w < 4294967296.0 &&
+    const int nhe = g.mx / 2;
+    return nhe % np == 0 && (ty + 2) * (nhe / np) <= 1024 && (g.nx + 1) / 2 <= 258 &&
+           pair_sweep_lds(g, ty, w, gc, np) <= 160 * 1024 && (double)g.n * w < 4294967296.0 &&
            (gc == nullptr || (double)gc->n * w < 4294967296.0);
 }
 
 bool pair_sweep_fits(const Geo& g, size_t w, const Geo* gc = nullptr) { return pair_sweep_rows_fit(g, 2, w, gc); }
 
-template <typename T, int TY, int NHM>
+template <typename T, int TY, int NHM, int NP>
 int pair_sweep_per_cu(const Geo& g)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jsv::k_rbsweep3p<T, TY, 1024, NHM>, 1024,
-                                                     pair_sweep_lds(g, TY, sizeof(T))) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jsv::k_rbsweep3p<T, TY, 1024, NHM, false, NP>, 1024,
+                                                     pair_sweep_lds(g, TY, sizeof(T), nullptr, NP)) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     return per_cu;
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int NP>
 int pair_sweep_per_cu(const Geo& g)
 {
-    return (g.nx + 1) / 2 <= 130 ? pair_sweep_per_cu<T, TY, 130>(g) : pair_sweep_per_cu<T, TY, 258>(g);
+    return (g.nx + 1) / 2 <= 130 ? pair_sweep_per_cu<T, TY, 130, NP>(g) : pair_sweep_per_cu<T, TY, 258, NP>(g);
 }
 
-template <typename T>
+template <typename T, int NP>
 int pair_sweep_per_cu(const Geo& g, int ty)
 {
     switch (ty) {
-    case 14: return pair_sweep_per_cu<T, 14>(g);
-    case 6: return pair_sweep_per_cu<T, 6>(g);
-    case 5: return pair_sweep_per_cu<T, 5>(g);
-    case 4: return pair_sweep_per_cu<T, 4>(g);
-    default: return pair_sweep_per_cu<T, 2>(g);
+    case 14: return pair_sweep_per_cu<T, 14, NP>(g);
+    case 6: return pair_sweep_per_cu<T, 6, NP>(g);
+    case 5: return pair_sweep_per_cu<T, 5, NP>(g);
+    case 4: return pair_sweep_per_cu<T, 4, NP>(g);
+    default: return pair_sweep_per_cu<T, 2, NP>(g);
     }
 }
 
-template <typename T, int TY, int NHM>
+template <typename T, int TY, int NHM, int NP>
 int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s)
 {
     const Geo& g = H.g[l];
     constexpr int NT = 1024;
     const int nh = (g.nx + 1) / 2;
     const Geo gc = pc != nullptr ? H.g[l + 1] : g;
-    if ((TY + 2) * (g.mx / 2) > NT || nh > NHM || !pair_sweep_rows_fit(g, TY, sizeof(T), pc ? &gc : nullptr))
+    if (nh > NHM || !pair_sweep_rows_fit(g, TY, sizeof(T), pc ? &gc : nullptr, NP))
         return fail("pair sweep does not fit level %d", l);
-    const size_t lds = pair_sweep_lds(g, TY, sizeof(T), pc ? &gc : nullptr);
+    const size_t lds = pair_sweep_lds(g, TY, sizeof(T), pc ? &gc : nullptr, NP);
     const int ytiles = (g.my - 1 + TY - 1) / TY;
     const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
     const dim3 grid((unsigned)(ytiles * zchunks));
     if (pc != nullptr)
-        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, true>), grid, dim3(NT), lds, s, H.phi[l], H.phi2[l],
-                           H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pc, gc);
+        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP>), grid, dim3(NT), lds, s, H.phi[l],
+                           H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pc, gc);
     else
-        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, false>), grid, dim3(NT), lds, s, H.phi[l], H.phi2[l],
-                           H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, (const T*)nullptr, gc);
+        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP>), grid, dim3(NT), lds, s, H.phi[l],
+                           H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles,
+                           (const T*)nullptr, gc);
     return 0;
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int NP>
 int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s)
 {
-    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130>(P, H, l, zchunk, pc, s);
-    return launch_rbsweep3p_t<T, TY, 258>(P, H, l, zchunk, pc, s);
+    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130, NP>(P, H, l, zchunk, pc, s);
+    return launch_rbsweep3p_t<T, TY, 258, NP>(P, H, l, zchunk, pc, s);
 }
 
-// Rows and z-chunk of the pair sweep.  Untuned, every fitting row count is
-// scored by the fraction of workgroup slots one wave of tiles fills times
-// the share of pair rows that are not halo, TY / (TY + 2); the z-chunk then
-// gives each slot one tile.
+template <typename T, int NP>
+int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zchunk, const T* pc, hipStream_t s)
+{
+    switch (rows) {
+    case 14: return launch_rbsweep3p_t<T, 14, NP>(P, H, l, zchunk, pc, s);
+    case 6: return launch_rbsweep3p_t<T, 6, NP>(P, H, l, zchunk, pc, s);
+    case 5: return launch_rbsweep3p_t<T, 5, NP>(P, H, l, zchunk, pc, s);
+    case 4: return launch_rbsweep3p_t<T, 4, NP>(P, H, l, zchunk, pc, s);
+    default: return launch_rbsweep3p_t<T, 2, NP>(P, H, l, zchunk, pc, s);
+    }
+}
+
+// Pairs per thread, rows and z-chunk of the pair sweep.  Two pairs per thread
+// in FP32 (fewer halo rows per useful row and twice the work between
+// barriers) where the registers allow it, one otherwise.  Untuned, every fitting row
+// count is scored by the fraction of workgroup slots one wave of tiles fills
+// times the share of pair rows that are not halo, TY / (TY + 2); the z-chunk
+// then gives each slot one tile.
 template <typename T>
 int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s)
 {
     const Geo& g = H.g[l];
     const Geo* gc = pc != nullptr ? &H.g[l + 1] : nullptr;
+    // default: two pairs in FP32 sweeps of levels >= 256 wide without the
+    // fused prolongation (with it, or in FP64, the two-pair kernel spills
+    // registers; on 128^3 one pair measured faster); tuning forces
+    int np = P->sweep_pairs > 0 ? P->sweep_pairs : ((sizeof(T) == 4 && pc == nullptr && g.mx >= 256) ? 2 : 1);
+    if (np == 2 && !pair_sweep_rows_fit(g, 2, sizeof(T), gc, 2)) np = 1;
     int rows = P->sweep_rows;
     if (rows == 8 || rows == 16) rows = 6;
-    const bool tuned = rows > 0 && pair_sweep_rows_fit(g, rows, sizeof(T), gc);
+    const bool tuned = rows > 0 && pair_sweep_rows_fit(g, rows, sizeof(T), gc, np);
     int zchunks = 0;
     if (!tuned) {
         rows = 0;
         double best = -1.0;
         for (int ty : {14, 6, 5, 4, 2}) {
-            if (!pair_sweep_rows_fit(g, ty, sizeof(T), gc)) continue;
-            const int slots = P->n_cu * pair_sweep_per_cu<T>(g, ty);
+            if (!pair_sweep_rows_fit(g, ty, sizeof(T), gc, np)) continue;
+            const int slots = P->n_cu * (np == 2 ? pair_sweep_per_cu<T, 2>(g, ty) : pair_sweep_per_cu<T, 1>(g, ty));
             const int ytiles = (g.my - 1 + ty - 1) / ty;
             int zc = slots / ytiles;
             if (zc < 1) zc = 1;
@@ -339,18 +362,14 @@ int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStrea
     if (zchunk <= 0) {
         if (zchunks <= 0) {
             const int ytiles = (g.my - 1 + rows - 1) / rows;
-            zchunks = P->n_cu * pair_sweep_per_cu<T>(g, rows) / ytiles;
+            zchunks = P->n_cu * (np == 2 ? pair_sweep_per_cu<T, 2>(g, rows) : pair_sweep_per_cu<T, 1>(g, rows)) /
+                      ytiles;
             if (zchunks < 1) zchunks = 1;
         }
         zchunk = (g.mz - 1 + zchunks - 1) / zchunks;
     }
-    switch (rows) {
-    case 14: return launch_rbsweep3p_t<T, 14>(P, H, l, zchunk, pc, s);
-    case 6: return launch_rbsweep3p_t<T, 6>(P, H, l, zchunk, pc, s);
-    case 5: return launch_rbsweep3p_t<T, 5>(P, H, l, zchunk, pc, s);
-    case 4: return launch_rbsweep3p_t<T, 4>(P, H, l, zchunk, pc, s);
-    default: return launch_rbsweep3p_t<T, 2>(P, H, l, zchunk, pc, s);
-    }
+    if (np == 2) return launch_rbsweep3p_rows<T, 2>(P, H, l, rows, zchunk, pc, s);
+    return launch_rbsweep3p_rows<T, 1>(P, H, l, rows, zchunk, pc, s);
 }
 
 // tile rows x threads per workgroup of the fused sweep (tunable; default by precision)
@@ -1026,6 +1045,11 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     if (key == JSV_TUNE_FUSE_PROLONG) {
         if (value > 2) return fail("fuse prolong must be 0, 1 or 2");
         P->fuse_prolong = (int)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_SWEEP_PAIRS) {
+        if (value > 2) return fail("sweep pairs must be 0, 1 or 2");
+        P->sweep_pairs = (int)value;
         return 0;
     }
     if (key == JSV_TUNE_SWEEP_VARIANT) {

@@ -370,7 +370,7 @@ k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict_
 
 // ---------------------------------------------------------------------------
 // Pair-per-thread variant of the fused sweep, with the coefficient stream
-// software-pipelined one plane ahead.
+// software-pipelined one plane ahead (NP consecutive pairs per thread).
 //
 // Same tiles, rings and step order as k_rbsweep3, but each thread owns one
 // column pair (x = 2k, 2k+1) of one ring row for the whole z-march.  In every
@@ -390,21 +390,20 @@ k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict_
 // sweep reads).  Requires (TY + 2) * (mx / 2) <= NT, (nx + 1) / 2 <= NHMAX
 // and the level below 4 GiB; rim values of phi_in are zero.
 // ---------------------------------------------------------------------------
-// two consecutive elements, loaded as one dword-aligned vector access
-template <typename T>
-struct __attribute__((aligned(sizeof(T)))) Pair2 {
-    T a, b;
+// N consecutive elements, loaded as one dword-aligned vector access
+template <typename T, int N>
+struct __attribute__((aligned(sizeof(T)))) PairVec {
+    T v[N];
 };
 
-template <typename T>
-struct PairCo {
-    T cxm, cxl, cxr;          // cx at x = 2k-1, 2k, 2k+1
-    T cyl, cyr, cyml, cymr;   // cy at (y, 2k), (y, 2k+1), (y-1, 2k), (y-1, 2k+1)
-    T czl, czr;               // cz at 2k, 2k+1
-    T rl, rr, ccl, ccr;       // rhs, diagonal at 2k, 2k+1
+// the coefficients of NP consecutive column pairs of one row and plane
+template <typename T, int NP>
+struct PairGrp {
+    T cxm;                     // cx at x = 2k - 1 (left of the first pair)
+    PairVec<T, 2 * NP> cx, cy, cym, cz, rh, dg;   // at x = 2k .. 2k + 2NP - 1; cym: row y - 1
 };
 
-template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int WPE = 1>
+template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int NP = 1, int WPE = 1>
 __global__ void __launch_bounds__(NT, WPE)
 k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
@@ -417,10 +416,11 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     constexpr int CMAX = (CRR * (NHMAX + 1) + NT - 1) / NT;   // coarse ring elements per thread
     const int nh = (g.nx + 1) >> 1;
     const int nhe = g.mx >> 1;                  // pairs per row
+    const int gpr = nhe / NP;                   // thread columns per row (NP pairs each)
     T* P = reinterpret_cast<T*>(jsv_dyn_smem);  // [3][PR][nh]
     T* R = P + 3 * PR * nh;                     // [3][RR][nh]
-    T* Hb = R + 3 * RR * nh;                    // [8][NT] this thread's held black half
-    T* Cr = Hb + 8 * NT;                        // PRO: [3][CRR][gc.nx] coarse planes of the correction
+    T* Hb = R + 3 * RR * nh;                    // [NP][8][NT] this thread's held black halves
+    T* Cr = Hb + 8 * NP * NT;                   // PRO: [3][CRR][gc.nx] coarse planes of the correction
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int y0 = 2 + (tile % ytiles) * TY;
@@ -431,9 +431,9 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
 
     for (int i = tid; i < 3 * RR * nh; i += NT) R[i] = (T)0;
 
-    const bool has = tid < RR * nhe;
-    const int r = has ? tid / nhe : 0;
-    const int k = has ? 1 + (tid - r * nhe) : 1;
+    const bool has = tid < RR * gpr;
+    const int r = has ? tid / gpr : 0;
+    const int k = has ? 1 + NP * (tid - r * gpr) : 1;   // first pair of this thread
     const int y = y0 - 1 + r;
     const bool ld = has && y < g.ny;
     const bool unk = has && y >= 2 && y <= g.my;
@@ -579,30 +579,19 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     auto Rv = [&](int z, int yy, int x) -> T {
         return R[((z % 3) * RR + (yy - y0 + 1)) * nh + (x >> 1)];
     };
-    auto at2 = [](const T* base, unsigned boff) -> Pair2<T> {
-        return *reinterpret_cast<const Pair2<T>*>(reinterpret_cast<const char*>(base) + boff);
+    using V = PairVec<T, 2 * NP>;
+    auto atv = [](const T* base, unsigned boff) -> V {
+        return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + boff);
     };
-    auto load = [&](int z, PairCo<T>& c) {
+    auto load = [&](int z, PairGrp<T, NP>& c) {
         const unsigned q = (unsigned)z * szb + rowb;
         c.cxm = at(cx, q - W);
-        Pair2<T> v = at2(cx, q);
-        c.cxl = v.a;
-        c.cxr = v.b;
-        v = at2(cy, q);
-        c.cyl = v.a;
-        c.cyr = v.b;
-        v = at2(cy, q - syb);
-        c.cyml = v.a;
-        c.cymr = v.b;
-        v = at2(cz, q);
-        c.czl = v.a;
-        c.czr = v.b;
-        v = at2(rhs, q);
-        c.rl = v.a;
-        c.rr = v.b;
-        v = at2(cc, q);
-        c.ccl = v.a;
-        c.ccr = v.b;
+        c.cx = atv(cx, q);
+        c.cy = atv(cy, q);
+        c.cym = atv(cy, q - syb);
+        c.cz = atv(cz, q);
+        c.rh = atv(rhs, q);
+        c.dg = atv(cc, q);
     };
 
     int chave = 0;                              // PRO: highest coarse plane in the ring
@@ -615,20 +604,17 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     load_plane(z0 - 1, corrected);
     issue(z0);
     prep_corr(z0);
-    PairCo<T> A, B;
+    PairGrp<T, NP> A, B;
     load(z0 - 1, A);
-    T czpl, czpr;                               // cz of the plane below the current one
-    {
-        const unsigned q = (unsigned)(z0 - 2) * szb + rowb;
-        czpl = at(cz, q);
-        czpr = at(cz, q + W);
-    }
-    // the black half of the previous plane (its update runs one step later)
-    // waits in LDS, private to the thread: no barrier needed
-    auto H = [&](int j) -> T& { return Hb[j * NT + tid]; };
-    T red_prev = 0;
+    V czp = atv(cz, (unsigned)(z0 - 2) * szb + rowb);   // cz of the plane below the current one
+    // the black halves of the previous plane (their updates run one step
+    // later) wait in LDS, private to the thread: no barrier needed
+    auto H = [&](int pp, int j) -> T& { return Hb[(pp * 8 + j) * NT + tid]; };
+    T red_prev[NP];
+#pragma unroll
+    for (int pp = 0; pp < NP; ++pp) red_prev[pp] = (T)0;
 
-    auto step = [&](int zs, PairCo<T>& cur, PairCo<T>& nxt) {
+    auto step = [&](int zs, PairGrp<T, NP>& cur, PairGrp<T, NP>& nxt) {
         // unconditional issue: planes up to z1 + 2 <= nz are requested (the
         // ring's guard zeroes plane nz; coefficient plane zs + 1 <= nz - 1)
         commit(zs + 1);
@@ -645,61 +631,79 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         issue(zs + 2);
         load(zs + 1, nxt);
         __syncthreads();
-        // ---- red on plane zs (x + y + z odd)
-        const bool left = ((y + zs) & 1) != 0;  // x = 2k is red
-        const int xr = 2 * k + (left ? 0 : 1);
-        T red = (T)0;
-        if (unk && zs >= 2 && zs <= g.mz && xr <= g.mx) {
-            const T a_zp = left ? cur.czl : cur.czr;
-            const T a_zm = left ? czpl : czpr;
-            const T a_yp = left ? cur.cyl : cur.cyr;
-            const T a_ym = left ? cur.cyml : cur.cymr;
-            const T a_xp = left ? cur.cxl : cur.cxr;
-            const T a_xm = left ? cur.cxm : cur.cxl;
-            // all ring reads first, then the reference's sum order
-            const T p_zp = Pv(zs + 1, y, xr), p_zm = Pv(zs - 1, y, xr);
-            const T p_yp = Pv(zs, y + 1, xr), p_ym = Pv(zs, y - 1, xr);
-            const T p_xp = Pv(zs, y, xr + 1), p_xm = Pv(zs, y, xr - 1);
-            T acc = a_zp * p_zp + a_zm * p_zm;
-            acc = acc + a_yp * p_yp;
-            acc = acc + a_ym * p_ym;
-            acc = acc + a_xp * p_xp;
-            acc = acc + a_xm * p_xm;
-            acc = acc - (left ? cur.rl : cur.rr);
-            red = acc / (left ? cur.ccl : cur.ccr);
+        // ---- red on plane zs (x + y + z odd); pair pp covers x = 2(k+pp), 2(k+pp)+1
+        const bool left = ((y + zs) & 1) != 0;  // the even x of each pair is red
+        const int e = left ? 0 : 1;             // red element within a pair
+        // element of pair pp: e ? odd : even (selects between constant indices;
+        // a runtime index would move the vectors to scratch memory)
+#define JSV_EL(vec, pp, odd) ((odd) ? (vec).v[2 * (pp) + 1] : (vec).v[2 * (pp)])
+#define JSV_XM(c, pp, odd) ((odd) ? (c).cx.v[2 * (pp)] : ((pp) == 0 ? (c).cxm : (c).cx.v[2 * (pp) - 1]))
+        T red[NP];
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp) {
+            const int xr = 2 * (k + pp) + e;
+            red[pp] = (T)0;
+            if (unk && zs >= 2 && zs <= g.mz && xr <= g.mx) {
+                const T a_zp = JSV_EL(cur.cz, pp, e);
+                const T a_zm = JSV_EL(czp, pp, e);
+                const T a_yp = JSV_EL(cur.cy, pp, e);
+                const T a_ym = JSV_EL(cur.cym, pp, e);
+                const T a_xp = JSV_EL(cur.cx, pp, e);
+                const T a_xm = JSV_XM(cur, pp, e);
+                // all ring reads first, then the reference's sum order
+                const T p_zp = Pv(zs + 1, y, xr), p_zm = Pv(zs - 1, y, xr);
+                const T p_yp = Pv(zs, y + 1, xr), p_ym = Pv(zs, y - 1, xr);
+                const T p_xp = Pv(zs, y, xr + 1), p_xm = Pv(zs, y, xr - 1);
+                T acc = a_zp * p_zp + a_zm * p_zm;
+                acc = acc + a_yp * p_yp;
+                acc = acc + a_ym * p_ym;
+                acc = acc + a_xp * p_xp;
+                acc = acc + a_xm * p_xm;
+                acc = acc - JSV_EL(cur.rh, pp, e);
+                red[pp] = acc / JSV_EL(cur.dg, pp, e);
+            }
+            if (has) R[((zs % 3) * RR + r) * nh + k + pp] = red[pp];
         }
-        if (has) R[((zs % 3) * RR + r) * nh + k] = red;
         __syncthreads();
-        // ---- black on plane zb = zs - 1 at x = xr; write both colours of the pair
+        // ---- black on plane zb = zs - 1 at x = xr; write both colours of the pairs
         const int zb = zs - 1;
         if (inner && zb >= z0 && zb < z1) {
             T* orow = pout + ((long)zb * sz + (long)y * sy);
-            if (xr <= g.mx) {
-                const T r_zp = Rv(zb + 1, y, xr), r_zm = Rv(zb - 1, y, xr);
-                const T r_yp = Rv(zb, y + 1, xr), r_ym = Rv(zb, y - 1, xr);
-                const T r_xp = Rv(zb, y, xr + 1), r_xm = Rv(zb, y, xr - 1);
-                T acc = H(0) * r_zp + H(1) * r_zm;
-                acc = acc + H(2) * r_yp;
-                acc = acc + H(3) * r_ym;
-                acc = acc + H(4) * r_xp;
-                acc = acc + H(5) * r_xm;
-                acc = acc - H(6);
-                orow[xr] = acc / H(7);
+#pragma unroll
+            for (int pp = 0; pp < NP; ++pp) {
+                const int xr = 2 * (k + pp) + e;
+                if (xr <= g.mx) {
+                    const T r_zp = Rv(zb + 1, y, xr), r_zm = Rv(zb - 1, y, xr);
+                    const T r_yp = Rv(zb, y + 1, xr), r_ym = Rv(zb, y - 1, xr);
+                    const T r_xp = Rv(zb, y, xr + 1), r_xm = Rv(zb, y, xr - 1);
+                    T acc = H(pp, 0) * r_zp + H(pp, 1) * r_zm;
+                    acc = acc + H(pp, 2) * r_yp;
+                    acc = acc + H(pp, 3) * r_ym;
+                    acc = acc + H(pp, 4) * r_xp;
+                    acc = acc + H(pp, 5) * r_xm;
+                    acc = acc - H(pp, 6);
+                    orow[xr] = acc / H(pp, 7);
+                }
+                if ((xr ^ 1) <= g.mx) orow[xr ^ 1] = red_prev[pp];
             }
-            if ((xr ^ 1) <= g.mx) orow[xr ^ 1] = red_prev;
         }
-        // ---- keep plane zs's black half (the point that is not red)
-        H(0) = left ? cur.czr : cur.czl;
-        H(1) = left ? czpr : czpl;
-        H(2) = left ? cur.cyr : cur.cyl;
-        H(3) = left ? cur.cymr : cur.cyml;
-        H(4) = left ? cur.cxr : cur.cxl;
-        H(5) = left ? cur.cxl : cur.cxm;
-        H(6) = left ? cur.rr : cur.rl;
-        H(7) = left ? cur.ccr : cur.ccl;
-        red_prev = red;
-        czpl = cur.czl;
-        czpr = cur.czr;
+        // ---- keep plane zs's black halves (the points that are not red)
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp) {
+            const int b1 = 1 - e;               // black element within the pair
+            H(pp, 0) = JSV_EL(cur.cz, pp, b1);
+            H(pp, 1) = JSV_EL(czp, pp, b1);
+            H(pp, 2) = JSV_EL(cur.cy, pp, b1);
+            H(pp, 3) = JSV_EL(cur.cym, pp, b1);
+            H(pp, 4) = JSV_EL(cur.cx, pp, b1);
+            H(pp, 5) = JSV_XM(cur, pp, b1);
+            H(pp, 6) = JSV_EL(cur.rh, pp, b1);
+            H(pp, 7) = JSV_EL(cur.dg, pp, b1);
+            red_prev[pp] = red[pp];
+        }
+#undef JSV_EL
+#undef JSV_XM
+        czp = cur.cz;
         // coarse planes up to cmax(zs + 2) are in the ring since this step's
         // first barrier; ring slot (zs + 2) % 3 last held plane zs - 1, read
         // by this step's red half before the second barrier

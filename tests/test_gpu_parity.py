@@ -96,13 +96,15 @@ def test_restrict_bitexact(case):
         np.testing.assert_array_equal(got, want, err_msg=f"level {level}")
 
 
-FUSED_VARIANTS = [(1, 0), (2, 0), (2, 2), (2, 4), (2, 5), (2, 6), (2, 14)]
+FUSED_VARIANTS = [(1, 0, 0), (2, 0, 1), (2, 2, 1), (2, 4, 1), (2, 5, 1), (2, 6, 1), (2, 14, 1),
+                  (2, 0, 2), (2, 2, 2), (2, 5, 2), (2, 14, 2)]
 
 
-@pytest.mark.parametrize("variant,rows", FUSED_VARIANTS, ids=[f"v{v}_rows{r}" for v, r in FUSED_VARIANTS])
+@pytest.mark.parametrize("variant,rows,pairs", FUSED_VARIANTS,
+                         ids=[f"v{v}_rows{r}_pairs{p}" for v, r, p in FUSED_VARIANTS])
 @pytest.mark.parametrize("zchunk", [1, 2, 3, 5, 0])
 @pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 8), (8, 8, 8)])
-def test_fused_sweep_bitexact(ms, zchunk, variant, rows):
+def test_fused_sweep_bitexact(ms, zchunk, variant, rows, pairs):
     """The fused out-of-place z-marching sweep (forced on every level, several
     y-tiles and z-chunk sizes, both kernels) gives the oracle's bits for
     sweeps and cycles."""
@@ -114,6 +116,7 @@ def test_fused_sweep_bitexact(ms, zchunk, variant, rows):
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_ZCHUNK, zchunk))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_VARIANT, variant))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_ROWS, rows))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_PAIRS, pairs))
     for level in range(op.levels - 1):
         phi, rhs = _level_arrays(op, level, 80 + level)
         dp.upload(level, 0, phi)
@@ -131,10 +134,11 @@ def test_fused_sweep_bitexact(ms, zchunk, variant, rows):
         np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle))
 
 
+@pytest.mark.parametrize("pairs", [1, 2])
 @pytest.mark.parametrize("fuse", [2, 0])
 @pytest.mark.parametrize("rows", [0, 2, 5])
 @pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 16), (8, 8, 8), (32, 32, 64)])
-def test_fused_prolongation_cycles_bitexact(ms, rows, fuse):
+def test_fused_prolongation_cycles_bitexact(ms, rows, fuse, pairs):
     """Post-smoothing pair sweeps that apply the prolongation themselves (a
     coarse-plane ring in LDS) give the oracle's bits for whole cycles: V, W,
     F, nu2 = 2 and 3, skipped pre-smoothing."""
@@ -146,6 +150,7 @@ def test_fused_prolongation_cycles_bitexact(ms, rows, fuse):
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_VARIANT, 2))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_ROWS, rows))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_PROLONG, fuse))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_PAIRS, pairs))
     for kw in (dict(), dict(gamma=2), dict(fcycle=True), dict(nu1=1, nu2=2), dict(gamma=2, nu1=2, nu2=3),
                dict(skip_pre=True)):
         dp.upload(0, 0, phi0)
@@ -159,21 +164,24 @@ def test_fused_prolongation_cycles_bitexact(ms, rows, fuse):
 @pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 16)])
 def test_fused_prolongation_fp32_matches_separate(ms):
     """FP32 cycles (the mixed solver's inner cycles): the fused prolongation
-    (default for FP32) gives the same bits as the separate prolongation pass."""
+    (default for FP32) and the two-pairs-per-thread kernel give the same bits
+    as the separate prolongation pass and one pair per thread."""
     from jsolver_amd import _lib
     phi, rhs, lam, D, _ = rand_problem(3, ms, seed=37)
     outs = []
-    for fuse in (1, 0):
+    for fuse, pairs in ((1, 0), (0, 1), (1, 2), (0, 2)):
         dp = DevPlan(3, ms, lam, D, precision="mixed")
         lib = _lib.load()
         _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSED_MIN_M, 2))
         _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_PROLONG, fuse))
+        _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_SWEEP_PAIRS, pairs))
         dp.upload(0, 0, gmg.zero_rim(phi).astype(np.float32), prec=_lib.JSV_MIXED)
         dp.upload(0, 1, rhs.astype(np.float32), prec=_lib.JSV_MIXED)
         for kw in (dict(), dict(gamma=2, nu2=2)):
             dp.cycle(cfg(**kw), prec=_lib.JSV_MIXED)
         outs.append(dp.download(0, 0, prec=_lib.JSV_MIXED))
-    np.testing.assert_array_equal(outs[0], outs[1])
+    for o in outs[1:]:
+        np.testing.assert_array_equal(outs[0], o)
     assert np.abs(outs[0]).max() > 0
 
 
