@@ -110,7 +110,8 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *   JSV_TUNE_COARSE_M     levels whose largest resolution is <= this (and
  *                         every coarser level) run each coarse sub-cycle as
  *                         one single-workgroup launch per problem
- *                         (default 16 in 3D, 64 in 2D; 0 = off)
+ *                         (default 3D: 8 for FP64, 16 for FP32; 2D: 64;
+ *                         0 = off; the tuned value applies to both)
  *   JSV_TUNE_COARSE_LDS   1: that launch keeps its coarsest levels (up to
  *                         150 KB) in LDS; 0: all levels in HBM (default 1 in
  *                         3D, 0 in 2D)
