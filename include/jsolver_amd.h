@@ -121,12 +121,12 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         mx <= 512, rows 2, 4, 5, 6 or 14; others use 1);
  *                         1 one thread per ring point
  *   JSV_TUNE_SWEEP_PAIRS  column pairs per thread of the pair kernel: 0
- *                         (default) 2 for FP32 sweeps of levels with mx >= 256
- *                         without the fused prolongation, else 1; 1; 2
+ *                         (default) 2 for FP32 sweeps of levels with mx >= 256,
+ *                         else 1; 1; 2
  *   JSV_TUNE_FUSE_PROLONG in 3D the first post-smoothing pair sweep of a
  *                         level applies the prolongation from the coarser
- *                         level itself: 1 (default) FP32 levels only, 2 FP32
- *                         and FP64, 0 never (separate prolongation pass) */
+ *                         level itself: 2 (default) FP32 and FP64 levels, 1
+ *                         FP32 levels only, 0 never (separate pass) */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2

@@ -120,7 +120,7 @@ struct jsv_plan {
     int sweep_variant = 2;          // 1: thread per ring point; 2: thread per column pair, pipelined
     int sweep_pairs = 0;            // pair kernel: column pairs per thread (0: 2 in FP32, 1 in FP64)
     int n_cu = 256;                 // compute units of the device
-    int fuse_prolong = 1;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
+    int fuse_prolong = 2;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
     bool use_graphs = true;         // replay whole cycles as captured hipGraphs
     int coarse_m[2] = {0, 0};       // FP64 / FP32: levels with max resolution <= this run as one coarse program (0: off)
     void* lv_dev[2] = {nullptr, nullptr};          // LvPtr<double> / LvPtr<float> per level
@@ -328,10 +328,10 @@ int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStrea
 {
     const Geo& g = H.g[l];
     const Geo* gc = pc != nullptr ? &H.g[l + 1] : nullptr;
-    // default: two pairs in FP32 sweeps of levels >= 256 wide without the
-    // fused prolongation (with it, or in FP64, the two-pair kernel spills
-    // registers; on 128^3 one pair measured faster); tuning forces
-    int np = P->sweep_pairs > 0 ? P->sweep_pairs : ((sizeof(T) == 4 && pc == nullptr && g.mx >= 256) ? 2 : 1);
+    // default: two pairs in FP32 sweeps of levels >= 256 wide (in FP64 the
+    // two-pair kernel spills registers; on 128^3 one pair measured faster);
+    // tuning forces
+    int np = P->sweep_pairs > 0 ? P->sweep_pairs : ((sizeof(T) == 4 && g.mx >= 256) ? 2 : 1);
     if (np == 2 && !pair_sweep_rows_fit(g, 2, sizeof(T), gc, 2)) np = 1;
     int rows = P->sweep_rows;
     if (rows == 8 || rows == 16) rows = 6;
@@ -397,8 +397,6 @@ int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T
 template <typename T>
 bool prolong_fusable(const jsv_plan* P, const Hier<T>& H, int l)
 {
-    // FP64 is off by default: the coarse-ring code makes the FP64 kernel
-    // spill registers and it loses more than the separate pass costs
     if (P->dim != 3 || P->fuse_prolong == 0 || (sizeof(T) == 8 && P->fuse_prolong < 2) || l + 1 >= P->levels)
         return false;
     const Geo& g = H.g[l];

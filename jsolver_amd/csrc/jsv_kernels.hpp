@@ -511,7 +511,8 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         if constexpr (PRO) {
             T* dst = P + (z % 3) * PR * nh;
             cpr_ok = 0;
-#pragma unroll
+            // not unrolled: one element's coarse reads live at a time
+#pragma unroll 1
             for (int j = 0; j < PMAX; ++j) {
                 const int i = tid + j * NT;
                 T c;
