@@ -468,8 +468,9 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
     return 0;
 }
 
+// zphi: the coarse phi to zero in the same kernels (nullptr: left alone)
 template <typename T>
-int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
+int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nullptr)
 {
     const Geo& gf = H.g[l];
     const Geo& gc = H.g[l + 1];
@@ -483,12 +484,12 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
                                P->offdiag ? H.drr[l] : nullptr, P->offdiag ? H.dlr[l] : nullptr, gf,
                                P->act_ptr());
             hipLaunchKernelGGL(jsv::k_restrict_gather2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
-                               H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr());
+                               H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr(), zphi);
         } else {
             hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
                                H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf);
             hipLaunchKernelGGL(jsv::k_restrict_gather3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
-                               H.rhs[l + 1], H.res[l], gf, gc);
+                               H.rhs[l + 1], H.res[l], gf, gc, zphi);
         }
         return 0;
     }
@@ -496,10 +497,10 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
         hipLaunchKernelGGL(jsv::k_restrict2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
                            H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cc[l],
                            P->offdiag ? H.drr[l] : nullptr, P->offdiag ? H.dlr[l] : nullptr, gf, gc,
-                           P->act_ptr());
+                           P->act_ptr(), zphi);
     else
         hipLaunchKernelGGL(jsv::k_restrict3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
-                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, gc);
+                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, gc, zphi);
     return 0;
 }
 
@@ -680,9 +681,10 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
     }
     if (!skip_pre)
         for (int i = 0; i < c.nu1; ++i) RETCHK(smooth(P, H, l, s));
-    RETCHK(restrict_(P, H, l, s));
     const int start = coarse_start(P, H);
     const jsv_plan::CoarseProg* pr = (start == l + 1) ? find_prog<T>(P, c, fcyc, start) : nullptr;
+    // phi of level l + 1 := 0 happens inside the restriction (or the program)
+    RETCHK(restrict_(P, H, l, s, pr != nullptr ? (T*)nullptr : H.phi[l + 1]));
     if (pr != nullptr) {
         // zero + the gamma (F: 2 gamma) sub-cycles of level l + 1 in one launch
         const int pi = sizeof(T) == 4 ? 1 : 0;
@@ -697,7 +699,6 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
             hipLaunchKernelGGL((jsv::k_coarse_prog<T, false>), grid, dim3(jsv::COARSE_THREADS), 0, s, lvp, pr->ops,
                                pr->nops, P->dim, P->act_ptr(), lf, lt, -1);
     } else {
-        RETCHK(zero(H.phi[l + 1], H.g[l + 1].n, s));
         if (fcyc)
             for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, true, false, s));
         for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, false, false, s));

@@ -803,12 +803,28 @@ __device__ __forceinline__ T restrict3_at(const T* __restrict__ phi, const T* __
     return (T)0.03125 * e + (T)0.125 * rcen;
 }
 
+// Zero the coarse phi that the cycle visits next, in the restriction kernel
+// instead of a separate fill: the thread of coarse unknown (I, J, K) clears it
+// and the rim points whose nearest unknown it is, so the whole array ends zero.
+template <typename T>
+__device__ __forceinline__ void zero_with_rim(T* __restrict__ a, const Geo& gc, long base, int dim, int I,
+                                              int J, int K)
+{
+    const int z_lo = (dim == 3 && I == 2) ? 0 : I, z_hi = (dim == 3 && I == gc.mz) ? gc.mz + 2 : I;
+    const int y_lo = J == 2 ? 0 : J, y_hi = J == gc.my ? gc.my + 2 : J;
+    const int x_lo = K == 2 ? 0 : K, x_hi = K == gc.mx ? gc.mx + 2 : K;
+    for (int z = z_lo; z <= z_hi; ++z)
+        for (int y = y_lo; y <= y_hi; ++y)
+            for (int x = x_lo; x <= x_hi; ++x)
+                a[base + (dim == 3 ? (long)z * gc.sz : 0) + (long)y * gc.sy + x] = (T)0;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_restrict2(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cc,
             const T* __restrict__ drr, const T* __restrict__ dlr, Geo gf, Geo gc,
-            const int* __restrict__ act)
+            const int* __restrict__ act, T* __restrict__ zphi)
 {
     const int b = blockIdx.z;
     if (act != nullptr && act[b] == 0) return;
@@ -817,13 +833,14 @@ k_restrict2(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__
     if (J > gc.my || K > gc.mx) return;
     const long c = (long)b * gf.sz + (long)(2 * J - 1) * gf.sy + (2 * K - 1);
     rc[(long)b * gc.sz + (long)J * gc.sy + K] = restrict2_at(phi, rhs, cx, cy, cc, drr, dlr, c, gf.sy);
+    if (zphi != nullptr) zero_with_rim(zphi, gc, (long)b * gc.sz, 2, 0, J, K);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-            const T* __restrict__ cc, Geo gf, Geo gc)
+            const T* __restrict__ cc, Geo gf, Geo gc, T* __restrict__ zphi)
 {
     const int I = 2 + blockIdx.z;
     const int J = 2 + blockIdx.y * BY + threadIdx.y;
@@ -831,6 +848,7 @@ k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__
     if (J > gc.my || K > gc.mx) return;
     const long c = (long)(2 * I - 1) * gf.sz + (long)(2 * J - 1) * gf.sy + (2 * K - 1);
     rc[(long)I * gc.sz + (long)J * gc.sy + K] = restrict3_at(phi, rhs, cx, cy, cz, cc, c, gf.sy, gf.sz);
+    if (zphi != nullptr) zero_with_rim(zphi, gc, 0, 3, I, J, K);
 }
 
 // Two-pass restriction for large levels: pass 1 stores the residual at every
@@ -855,7 +873,7 @@ k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
 
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
-k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc)
+k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, T* __restrict__ zphi)
 {
     const int I = 2 + blockIdx.z;
     const int J = 2 + blockIdx.y * BY + threadIdx.y;
@@ -875,6 +893,7 @@ k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc)
     e = e + r[c - sz + sy];
     e = e + r[c - sz - sy];
     rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * r[c];
+    if (zphi != nullptr) zero_with_rim(zphi, gc, 0, 3, I, J, K);
 }
 
 template <typename T>
@@ -896,7 +915,7 @@ k_resid_first2(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_restrict_gather2(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc,
-                   const int* __restrict__ act)
+                   const int* __restrict__ act, T* __restrict__ zphi)
 {
     const int b = blockIdx.z;
     if (act != nullptr && act[b] == 0) return;
@@ -909,6 +928,7 @@ k_restrict_gather2(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc,
     s = s + r[c - sy + 1];
     s = s + r[c - sy - 1];
     rc[(long)b * gc.sz + (long)J * gc.sy + K] = (T)0.0625 * s + (T)0.25 * r[c];
+    if (zphi != nullptr) zero_with_rim(zphi, gc, (long)b * gc.sz, 2, 0, J, K);
 }
 
 // ---------------------------------------------------------------------------

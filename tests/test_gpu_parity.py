@@ -300,6 +300,29 @@ def test_cycle_coarse_program_operator_switch():
                                           err_msg=f"{kw} dxy={use_dxy}")
 
 
+@pytest.mark.parametrize("coarse_m", [0, 16])
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("2d_spatial_16x32", "2d_dxy_16", "3d_spatial_16x8x32")]
+                         + [("3d_spatial_32", 3, (32, 32, 32), True, True, False),
+                            ("2d_spatial_128x64", 2, (128, 64), True, True, False)], ids=lambda c: c[0])
+def test_cycle_clears_stale_coarse_phi(case, coarse_m):
+    """Coarse phi arrays full of junk (rims included) before a cycle: the
+    restriction's zeroing of the next level's phi must clear every element,
+    so the cycle still gives the oracle's bits."""
+    from jsolver_amd import _lib
+    dp, op, phi, rhs = _setup(case, seed=41)
+    _lib.check(_lib.load().jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, coarse_m))
+    rng = np.random.default_rng(5)
+    for kw in (dict(), dict(gamma=2, fcycle=True)):
+        for level in range(1, op.levels):
+            dp.upload(level, 0, rng.standard_normal(op.lv[level].shape))
+        dp.upload(0, 0, phi)
+        dp.upload(0, 1, rhs)
+        dp.cycle(cfg(**kw))
+        c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+        want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
+        np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=str(kw))
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_residual_matches_oracle(case):
     dp, op, phi, rhs = _setup(case, seed=7)
