@@ -112,9 +112,8 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         one single-workgroup launch per problem
  *                         (default 3D: 8 for FP64, 16 for FP32; 2D: 64;
  *                         0 = off; the tuned value applies to both)
- *   JSV_TUNE_COARSE_LDS   1: that launch keeps its coarsest levels (up to
- *                         150 KB) in LDS; 0: all levels in HBM (default 1 in
- *                         3D, 0 in 2D)
+ *   JSV_TUNE_COARSE_LDS   1 (default): that launch keeps its coarsest levels
+ *                         (up to 150 KB) in LDS; 0: all levels in HBM
  *   JSV_TUNE_SWEEP_VARIANT kernel of the fused sweep: 2 (default, 0) one
  *                         thread per red/black column pair with the
  *                         coefficients loaded one plane ahead (levels with
@@ -126,7 +125,12 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *   JSV_TUNE_FUSE_PROLONG in 3D the first post-smoothing pair sweep of a
  *                         level applies the prolongation from the coarser
  *                         level itself: 2 (default) FP32 and FP64 levels, 1
- *                         FP32 levels only, 0 never (separate pass) */
+ *                         FP32 levels only, 0 never (separate pass)
+ *   JSV_TUNE_COARSE_SOLO  ops of that launch on LDS levels touching at most
+ *                         this many points run in one wave without workgroup
+ *                         barriers (default 64, at most 64; 0 = never)
+ *   JSV_TUNE_COARSE_THREADS workgroup size of that launch: 256, 512 or 1024
+ *                         (default 1024) */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -139,6 +143,8 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_FUSE_PROLONG 9
 #define JSV_TUNE_COARSE_LDS 10
 #define JSV_TUNE_SWEEP_PAIRS 11
+#define JSV_TUNE_COARSE_SOLO 12
+#define JSV_TUNE_COARSE_THREADS 13
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

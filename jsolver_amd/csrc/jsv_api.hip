@@ -125,12 +125,19 @@ struct jsv_plan {
     int coarse_m[2] = {0, 0};       // FP64 / FP32: levels with max resolution <= this run as one coarse program (0: off)
     void* lv_dev[2] = {nullptr, nullptr};          // LvPtr<double> / LvPtr<float> per level
     std::vector<const void*> lv_host[2];           // pointers lv_dev holds (start level + table)
-    int coarse_lds = 1;             // stage the program's coarsest levels in LDS (set for 3D at creation)
+    int coarse_lds = 1;             // stage the program's coarsest levels in LDS
+    int coarse_solo = jsv::COARSE_WAVE;   // ops on staged levels with at most this many points run in one wave
+    int coarse_threads = jsv::COARSE_THREADS;   // workgroup size of the coarse program
     int clds_from[2] = {0, 0}, clds_to[2] = {0, 0};   // staged level range per precision
     size_t clds_bytes[2] = {0, 0};
+    void* img_dev[2] = {nullptr, nullptr};   // coefficient image of the staged levels (per problem)
+    size_t img_cap[2] = {0, 0};              // its capacity in bytes
+    int img_n[2] = {0, 0};                   // elements per problem
+    std::vector<const void*> img_key[2];     // layout and operator version the image holds
+    long op_version = 0;                     // bumped by every jsv_plan_set_operator
     struct CoarseProg {
-        int prec, gamma, nu1, nu2, fcyc, start, offdiag;
-        jsv::CoarseOp* ops;
+        int prec, gamma, nu1, nu2, fcyc, start, offdiag, lds_from, solo;
+        unsigned* ops;              // packed op words; nullptr: too long, the cycle recurses instead
         int nops;
     };
     std::vector<CoarseProg> progs;
@@ -576,8 +583,7 @@ void build_ops(const jsv_plan* P, const jsv_config& c, int l, bool fcyc, std::ve
         return;
     }
     for (int i = 0; i < c.nu1; ++i) smooth_ops(l);
-    ops.push_back({jsv::CO_RESTRICT, l, 0, 0});
-    ops.push_back({jsv::CO_ZERO, l + 1, 0, 0});
+    ops.push_back({jsv::CO_RESTRICT, l, 0, 1});      // also phi of l + 1 := 0
     if (fcyc)
         for (int i = 0; i < c.gamma; ++i) build_ops(P, c, l + 1, true, ops);
     for (int i = 0; i < c.gamma; ++i) build_ops(P, c, l + 1, false, ops);
@@ -590,11 +596,37 @@ template <typename T>
 const jsv_plan::CoarseProg* find_prog(const jsv_plan* P, const jsv_config& c, bool fcyc, int start)
 {
     const int prec = sizeof(T) == 4 ? JSV_MIXED : JSV_FP64;
+    const int pi = sizeof(T) == 4 ? 1 : 0;
     for (const auto& pr : P->progs)
         if (pr.prec == prec && pr.gamma == c.gamma && pr.nu1 == c.nu1 && pr.nu2 == c.nu2 &&
-            pr.fcyc == (fcyc ? 1 : 0) && pr.start == start && pr.offdiag == (P->offdiag ? 1 : 0))
+            pr.fcyc == (fcyc ? 1 : 0) && pr.start == start && pr.offdiag == (P->offdiag ? 1 : 0) &&
+            pr.lds_from == P->clds_from[pi] && pr.solo == P->coarse_solo)
             return &pr;
     return nullptr;
+}
+
+// Loop count of one op in k_coarse_prog, for the one-wave runs: exact
+// extents on global levels, power-of-two extents on staged ones (whose arrays
+// are compact, ln elements).
+long op_points(const jsv_plan* P, const std::vector<Geo>& G, const std::vector<int>& ln, bool staged,
+               const jsv::CoarseOp& op)
+{
+    const Geo& g = G[op.level];
+    const int e = staged ? 0 : 1;
+    const long nz = P->dim == 3 ? g.mz - e : 1;
+    const long full = staged ? ln[op.level] : (P->dim == 3 ? g.n : g.sz);
+    switch (op.kind) {
+    case jsv::CO_RB: return (long)(g.mx / 2) * (g.my - e) * nz;
+    case jsv::CO_RB_DIAG: return (long)(g.mx / 2) * (g.my / 2);
+    case jsv::CO_ZERO: return full;
+    case jsv::CO_PROLONG: return (long)(g.mx - e) * (g.my - e) * nz;
+    default: {
+        const Geo& gc = G[op.level + 1];
+        const long inner = (long)(gc.mx - e) * (gc.my - e) * (P->dim == 3 ? gc.mz - e : 1);
+        const long zc = op.b ? (staged ? ln[op.level + 1] : (P->dim == 3 ? gc.n : gc.sz)) : 0;
+        return inner > zc ? inner : zc;
+    }
+    }
 }
 
 // Uploads the level table and the two op lists a cycle of configuration c
@@ -612,32 +644,54 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
     auto drr = [&](int l) -> T* { return P->offdiag ? H.drr[l] : nullptr; };
     auto dlr = [&](int l) -> T* { return P->offdiag ? H.dlr[l] : nullptr; };
     // LDS staging: the coarsest levels (not level L-1, which no cycle
-    // visits) from the bottom up while they fit 150 KB
-    int lds_from = P->levels - 1, lds_to = P->levels - 1;
+    // visits) from the bottom up while they fit.  Each staged level is one
+    // block of its arrays, each compact over indices 1..m+1 per axis; the
+    // whole region arrives as one image (coefficients, zero phi / rhs).
+    int lds_from = P->levels - 1;
+    const int lds_to = P->levels - 1;
+    auto has = [&](int j) { return j <= 3 || j == 5 || (j == 4 && P->dim == 3) || (j >= 6 && P->offdiag); };
+    std::vector<int> lsy(P->levels), lsz(P->levels), ln(P->levels);
+    for (int l = 0; l < P->levels; ++l) {
+        const Geo& g = H.g[l];
+        lsy[l] = g.mx + 1;
+        lsz[l] = lsy[l] * (g.my + 1);
+        ln[l] = P->dim == 3 ? lsz[l] * (g.mz + 1) : lsz[l];
+    }
+    // op lists of both programs (V/W and F entry) first: their LDS copy shares the budget
+    std::vector<jsv::CoarseOp> ops_of[2];
+    for (int fc = 0; fc < 2; ++fc) {
+        std::vector<jsv::CoarseOp>& ops = ops_of[fc];
+        ops.push_back({jsv::CO_ZERO, start, 0, 0, 0});
+        if (fc)
+            for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, true, ops);
+        for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, false, ops);
+    }
+    const size_t op_bytes = 4 * std::max(ops_of[0].size(), ops_of[1].size());
+    size_t nar = 0;
+    for (int j = 0; j < 8; ++j) nar += has(j) ? 1 : 0;
     size_t used = 0;
-    std::vector<std::array<int, 8>> loff(P->levels);
-    for (auto& a : loff) a.fill(-1);
-    if (P->coarse_lds) {
+    if (P->coarse_lds && op_bytes <= 4 * (size_t)jsv::COARSE_MAX_OPS) {
         for (int l = P->levels - 2; l >= start; --l) {
-            const Geo& g = H.g[l];
-            const size_t np = P->dim == 3 ? (size_t)g.n : (size_t)g.sz;
-            const bool has[8] = {true, true, true, true, P->dim == 3, true, P->offdiag, P->offdiag};
-            size_t nar = 0;
-            for (bool h : has) nar += h ? 1 : 0;
-            if (used + np * nar * sizeof(T) > 150 * 1024) break;
-            size_t o = used / sizeof(T);
-            for (int j = 0; j < 8; ++j)
-                if (has[j]) {
-                    loff[l][j] = (int)o;
-                    o += np;
-                }
-            used += np * nar * sizeof(T);
+            const size_t lb = (size_t)ln[l] * nar * sizeof(T);
+            if (used + lb + op_bytes > 156 * 1024) break;
+            used += lb;
             lds_from = l;
         }
     }
+    std::vector<std::array<int, 8>> loff(P->levels);
+    for (auto& a : loff) a.fill(-1);
+    int o = 0;
+    for (int l = lds_from; l < lds_to; ++l)
+        for (int j = 0; j < 8; ++j)
+            if (has(j)) {
+                loff[l][j] = o;
+                o += ln[l];
+            }
+    const int img_n = o;
     P->clds_from[pi] = lds_from;
     P->clds_to[pi] = lds_to;
     P->clds_bytes[pi] = used;
+    P->img_n[pi] = img_n;
     std::vector<const void*> key{(const void*)(long)start, (const void*)(long)lds_from};
     for (int l = start; l < P->levels; ++l)
         for (const T* q : {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l)})
@@ -645,7 +699,8 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
     if (P->lv_host[pi] != key) {
         std::vector<jsv::LvPtr<T>> lv(P->levels);
         for (int l = start; l < P->levels; ++l) {
-            lv[l] = {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l), H.g[l], {}};
+            lv[l] = {H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l), H.g[l], {},
+                     lsy[l], lsz[l], ln[l]};
             for (int j = 0; j < 8; ++j) lv[l].lds[j] = loff[l][j];
         }
         const size_t nb = sizeof(jsv::LvPtr<T>) * lv.size();
@@ -654,18 +709,65 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         HIPCHK(hipMemcpy(P->lv_dev[pi], lv.data(), nb, hipMemcpyHostToDevice));
         P->lv_host[pi] = key;
     }
+    // the image follows the coefficients: rebuilt when the operator or the layout changed
+    if (img_n > 0) {
+        std::vector<const void*> ikey = key;
+        ikey.push_back((const void*)P->op_version);
+        if (P->img_key[pi] != ikey) {
+            const size_t nb = sizeof(T) * (size_t)img_n * (size_t)P->batch;
+            if (P->img_cap[pi] < nb) {
+                HIPCHK(hipStreamSynchronize(s));
+                RETCHK(dalloc(P, &P->img_dev[pi], nb));
+                P->img_cap[pi] = nb;
+            }
+            T* img = (T*)P->img_dev[pi];
+            RETCHK(zero(img, (long)img_n * P->batch, s));      // phi and rhs slots
+            for (int l = lds_from; l < lds_to; ++l) {
+                const T* src[8] = {nullptr, nullptr, H.cx[l], H.cy[l], H.cz[l], H.cc[l], drr(l), dlr(l)};
+                const long sbs = P->dim == 2 ? H.g[l].sz : 0;
+                for (int j = 2; j < 8; ++j)
+                    if (has(j))
+                        hipLaunchKernelGGL(jsv::k_stage_image<T>,
+                                           dim3((unsigned)((ln[l] + 255) / 256), (unsigned)P->batch), dim3(256), 0,
+                                           s, img, (long)img_n, loff[l][j], src[j], sbs, H.g[l], lsy[l], lsz[l],
+                                           ln[l], P->dim);
+            }
+            LAUNCHCHK();
+            P->img_key[pi] = ikey;
+        }
+    }
     for (int fc = 0; fc < 2; ++fc) {
         if (find_prog<T>(P, c, fc != 0, start) != nullptr) continue;
-        std::vector<jsv::CoarseOp> ops;
-        ops.push_back({jsv::CO_ZERO, start, 0, 0});
-        if (fc)
-            for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, true, ops);
-        for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, false, ops);
-        jsv::CoarseOp* d = nullptr;
-        RETCHK(dalloc(P, (void**)&d, sizeof(jsv::CoarseOp) * ops.size()));
-        HIPCHK(hipMemcpy(d, ops.data(), sizeof(jsv::CoarseOp) * ops.size(), hipMemcpyHostToDevice));
+        std::vector<jsv::CoarseOp>& ops = ops_of[fc];
+        if (ops.size() > (size_t)jsv::COARSE_MAX_OPS) {
+            P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
+                                P->offdiag ? 1 : 0, lds_from, P->coarse_solo, nullptr, (int)ops.size()});
+            continue;
+        }
+        // runs of small ops on staged levels go to one wave
+        auto staged = [&](int l) { return l >= lds_from && l < lds_to; };
+        for (size_t i = 0; i < ops.size();) {
+            auto solo = [&](const jsv::CoarseOp& op) {
+                const bool two = op.kind == jsv::CO_RESTRICT || op.kind == jsv::CO_PROLONG;
+                return P->coarse_solo > 0 && staged(op.level) && (!two || staged(op.level + 1)) &&
+                       op_points(P, H.g, ln, true, op) <= P->coarse_solo;
+            };
+            size_t k = i;
+            while (k < ops.size() && solo(ops[k])) ++k;
+            if (k > i) {
+                ops[i].run = (int)(k - i);
+                i = k;
+            } else {
+                ++i;
+            }
+        }
+        std::vector<unsigned> words(ops.size());
+        for (size_t i = 0; i < ops.size(); ++i) words[i] = jsv::coarse_pack(ops[i]);
+        unsigned* d = nullptr;
+        RETCHK(dalloc(P, (void**)&d, sizeof(unsigned) * words.size()));
+        HIPCHK(hipMemcpy(d, words.data(), sizeof(unsigned) * words.size(), hipMemcpyHostToDevice));
         P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
-                            P->offdiag ? 1 : 0, d, (int)ops.size()});
+                            P->offdiag ? 1 : 0, lds_from, P->coarse_solo, d, (int)ops.size()});
     }
     return 0;
 }
@@ -683,6 +785,7 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
         for (int i = 0; i < c.nu1; ++i) RETCHK(smooth(P, H, l, s));
     const int start = coarse_start(P, H);
     const jsv_plan::CoarseProg* pr = (start == l + 1) ? find_prog<T>(P, c, fcyc, start) : nullptr;
+    if (pr != nullptr && pr->ops == nullptr) pr = nullptr;
     // phi of level l + 1 := 0 happens inside the restriction (or the program)
     RETCHK(restrict_(P, H, l, s, pr != nullptr ? (T*)nullptr : H.phi[l + 1]));
     if (pr != nullptr) {
@@ -691,13 +794,25 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
         const int lf = P->clds_from[pi], lt = P->clds_to[pi];
         const dim3 grid((unsigned)(P->dim == 2 ? P->batch : 1));
         const auto* lvp = (const jsv::LvPtr<T>*)P->lv_dev[pi];
-        if (lf < lt)
-            hipLaunchKernelGGL((jsv::k_coarse_prog<T, true>), grid, dim3(jsv::COARSE_THREADS), P->clds_bytes[pi], s,
-                               lvp, pr->ops, pr->nops, P->dim, P->act_ptr(), lf, lt,
-                               (lf <= start && start < lt) ? start : -1);
+        // LDS: the staged levels (the image), then the op words
+        const int ops_at = P->img_n[pi];
+        const size_t lds = P->clds_bytes[pi] + sizeof(unsigned) * (size_t)pr->nops;
+        const int wb = (lf <= start && start < lt) ? start : -1;
+        const T* img = (const T*)P->img_dev[pi];
+        const int imn = P->img_n[pi];
+        auto launch = [&](auto kern, int nt) {
+            hipLaunchKernelGGL(kern, grid, dim3(nt), lds, s, lvp, pr->ops, pr->nops, ops_at, P->act_ptr(), lf, lt,
+                               wb, img, imn, (long)imn);
+        };
+        const int nt = P->coarse_threads;
+        if (P->dim == 2)
+            nt == 256   ? launch(jsv::k_coarse_prog<T, 256, 2>, nt)
+            : nt == 512 ? launch(jsv::k_coarse_prog<T, 512, 2>, nt)
+                        : launch(jsv::k_coarse_prog<T, 1024, 2>, nt);
         else
-            hipLaunchKernelGGL((jsv::k_coarse_prog<T, false>), grid, dim3(jsv::COARSE_THREADS), 0, s, lvp, pr->ops,
-                               pr->nops, P->dim, P->act_ptr(), lf, lt, -1);
+            nt == 256   ? launch(jsv::k_coarse_prog<T, 256, 3>, nt)
+            : nt == 512 ? launch(jsv::k_coarse_prog<T, 512, 3>, nt)
+                        : launch(jsv::k_coarse_prog<T, 1024, 3>, nt);
     } else {
         if (fcyc)
             for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, true, false, s));
@@ -909,7 +1024,7 @@ int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double i
     // the FP32 cycle from 16^3
     P->coarse_m[0] = dim == 3 ? 8 : 64;
     P->coarse_m[1] = dim == 3 ? 16 : 64;
-    P->coarse_lds = dim == 3 ? 1 : 0;   // measured: helps 3D FP32, slows the 2D batched W-cycles
+    P->coarse_lds = 1;
     {
         int dev = 0, ncu = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -1041,6 +1156,16 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         P->lv_host[1].clear();
         return 0;
     }
+    if (key == JSV_TUNE_COARSE_SOLO) {
+        if (value > jsv::COARSE_WAVE) return fail("coarse solo limit must be <= %d points", jsv::COARSE_WAVE);
+        P->coarse_solo = (int)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_COARSE_THREADS) {
+        if (value != 256 && value != 512 && value != 1024) return fail("coarse program threads must be 256, 512 or 1024");
+        P->coarse_threads = (int)value;
+        return 0;
+    }
     if (key == JSV_TUNE_ZCHUNK) {
         P->zchunk = (int)value;
         return 0;
@@ -1097,6 +1222,7 @@ int jsv_plan_set_operator(jsv_plan* P, const double* lam, double lam_scalar, con
     if (!spatial && d_scalar == nullptr) return fail("scalar D values missing");
     if (dxy != nullptr && (dim != 2 || !spatial)) return fail("D_xy needs 2D array diffusion");
     P->spatial = spatial;
+    ++P->op_version;                                      // the coarse program's image is stale
     if (P->offdiag != (dxy != nullptr)) drop_graphs(P);   // the smoother's kernels change
     P->offdiag = dxy != nullptr;
     Hier<double>& H = P->h64;

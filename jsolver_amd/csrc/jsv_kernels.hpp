@@ -144,13 +144,12 @@ k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
 // the prolongation kernels and the pair sweep that applies the correction on the fly.
 // Correction at fine point (x, y[, z]) from coarse array pc (problem offset
 // included); false where the point is skipped (full == 0).
-template <typename T>
-__device__ __forceinline__ bool prolong2_corr(const T* __restrict__ pc, long csy, int x, int y,
-                                              int full, T& corr)
+// C2(J, K) reads the coarse correction at coarse point (J, K)
+template <typename T, typename Coarse>
+__device__ __forceinline__ bool prolong2_corr_at(Coarse C2, int x, int y, int full, T& corr)
 {
     const int ox = x & 1, oy = y & 1;
     const int Xo = (x + 1) >> 1, Yo = (y + 1) >> 1, Xl = x >> 1, Yl = y >> 1;
-#define C2(J, K) pc[(long)(J) * csy + (K)]
     if (ox && oy) {
         corr = C2(Yo, Xo);
     } else if (oy) {
@@ -164,8 +163,14 @@ __device__ __forceinline__ bool prolong2_corr(const T* __restrict__ pc, long csy
         s = s + C2(Yl, Xl);
         corr = (T)0.25 * s;
     }
-#undef C2
     return true;
+}
+
+template <typename T>
+__device__ __forceinline__ bool prolong2_corr(const T* __restrict__ pc, long csy, int x, int y,
+                                              int full, T& corr)
+{
+    return prolong2_corr_at<T>([&](int J, int K) -> T { return pc[(long)J * csy + K]; }, x, y, full, corr);
 }
 
 // C3(I, J, K) reads the coarse correction at coarse point (I, J, K)
@@ -973,9 +978,20 @@ k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int fu
 // separate launches each costs a launch plus a fill/drain of the whole GPU for
 // microseconds of work.  One workgroup per problem instead runs the whole
 // coarse sub-cycle from a host-built op list, with a workgroup barrier between
-// ops (the coarse levels stay resident in L2).  The per-point arithmetic is
-// the same device functions the launch-per-op kernels use, so results are
-// bit-identical.
+// ops.  The per-point arithmetic is the same device functions the
+// launch-per-op kernels use, so results are bit-identical.
+//
+// Latency, not bandwidth, bounds these ops, so the coarsest levels live in
+// LDS for the whole launch: levels [lds_from, lds_to) are addressed with LDS
+// instructions (each op body is instantiated per storage of the two levels it
+// touches, so the compiler sees LDS pointers, not generic ones).  Their
+// coefficients arrive in one pipelined copy from a prebuilt image that holds
+// them in LDS order; their rhs and phi are produced inside the program (the
+// first op zeroes the start level's phi; each restriction writes the next
+// level's rhs and zeroes its phi), except the start level's rhs, copied in,
+// and its phi, copied back at the end.  Runs of ops on LDS levels that touch
+// no more points than one wave has lanes execute in wave 0 alone, in order,
+// without workgroup barriers (LDS operations of one wave complete in order).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct LvPtr {
@@ -988,164 +1004,333 @@ struct LvPtr {
     const T* drr;
     const T* dlr;
     Geo g;
-    // element offsets of this level's arrays in the program's LDS copy
-    // (phi, rhs, cx, cy, cz, cc, drr, dlr; -1: array absent); used when the
-    // level lies in the staged range of the launch
+    // LDS copy of this level, used when the level lies in the staged range of
+    // the launch: element offsets of phi, rhs, cx, cy, cz, cc, drr, dlr (-1:
+    // absent), each a compact array over indices 1..m+1 per axis (the ghost
+    // layers 0 and m+2 are never read) of lsz elements per plane, lsy per row
+    // and ln in total
     int lds[8];
+    int lsy, lsz, ln;
 };
 
 enum CoarseOpKind : int {
     CO_RB = 0,        // one RBGS half: a = parity
     CO_RB_DIAG = 1,   // one D_xy sub-colour: a = lx, b = ly
-    CO_RESTRICT = 2,  // defect of `level` restricted into rhs of level + 1
+    CO_RESTRICT = 2,  // defect of `level` restricted into rhs of level + 1; b = 1: also phi of level + 1 := 0
     CO_ZERO = 3,      // phi of `level` := 0 (whole array incl. rim)
     CO_PROLONG = 4,   // phi of `level` += P phi of level + 1; a = full
 };
 
 struct CoarseOp {
     int kind, level, a, b;
+    int run;          // > 0: first of `run` consecutive ops that wave 0 runs alone
 };
 
+// An op as the kernel reads it: one 32-bit word (kind 3 bits, level 5, a 1,
+// b 1, run 22), so the whole list sits in LDS next to the levels.
+__host__ __device__ __forceinline__ unsigned coarse_pack(const CoarseOp& op)
+{
+    return (unsigned)op.kind | ((unsigned)op.level << 3) | ((unsigned)op.a << 8) | ((unsigned)op.b << 9) |
+           ((unsigned)op.run << 10);
+}
+__device__ __forceinline__ CoarseOp coarse_unpack(unsigned w)
+{
+    return {(int)(w & 7u), (int)((w >> 3) & 31u), (int)((w >> 8) & 1u), (int)((w >> 9) & 1u), (int)(w >> 10)};
+}
+constexpr int COARSE_MAX_OPS = 8192;   // longer programs run as separate launches
 constexpr int COARSE_THREADS = 1024;
+constexpr int COARSE_WAVE = 64;
 
-// Levels [lds_from, lds_to) are staged in LDS for the whole program (this
-// problem's coefficients and rhs copied in at the start); level wb_level's
-// phi is copied back at the end (-1: none).  Staged arrays are addressed
-// through generic pointers, so the same point functions serve both kinds.
-template <typename T, bool STAGED>
-__global__ void __launch_bounds__(COARSE_THREADS)
-k_coarse_prog(const LvPtr<T>* __restrict__ lv, const CoarseOp* __restrict__ ops, int nops, int dim,
-              const int* __restrict__ act, int lds_from, int lds_to, int wb_level)
+// Array pointers of one level and the map from padded coordinates to their
+// element index, p = z * sz + y * sy + x - org: the LDS copy (IN_LDS, compact
+// strides, org = the offset of point (1, 1, 1)) or the global arrays at
+// problem offset bo (padded strides, org = 0).  zb / zn: the whole array, for
+// zeroing.
+template <typename T, bool IN_LDS>
+struct LvView {
+    T* phi;
+    T* rhs;
+    const T* cx;
+    const T* cy;
+    const T* cz;
+    const T* cc;
+    const T* drr;
+    const T* dlr;
+    bool od;
+    long sy, sz, org;
+    int zn;
+    __device__ __forceinline__ LvView(const LvPtr<T>& L, T* sm, long bo, int dim)
+    {
+        if constexpr (IN_LDS) {
+            phi = sm + L.lds[0];
+            rhs = sm + L.lds[1];
+            cx = sm + L.lds[2];
+            cy = sm + L.lds[3];
+            cz = sm + (L.lds[4] >= 0 ? L.lds[4] : 0);
+            cc = sm + L.lds[5];
+            od = L.lds[6] >= 0;
+            drr = sm + (od ? L.lds[6] : 0);
+            dlr = sm + (od ? L.lds[7] : 0);
+            sy = L.lsy;
+            sz = L.lsz;
+            org = (dim == 3 ? sz : 0) + sy + 1;
+            zn = L.ln;
+        } else {
+            phi = L.phi + bo;
+            rhs = L.rhs + bo;
+            cx = L.cx + bo;
+            cy = L.cy + bo;
+            cz = L.cz != nullptr ? L.cz + bo : L.cx + bo;
+            cc = L.cc + bo;
+            od = L.drr != nullptr;
+            drr = od ? L.drr + bo : L.cx + bo;
+            dlr = od ? L.dlr + bo : L.cx + bo;
+            sy = L.g.sy;
+            sz = L.g.sz;
+            org = 0;
+            zn = (int)(dim == 2 ? L.g.sz : L.g.n);
+        }
+    }
+    __device__ __forceinline__ long at(int z, int y, int x) const { return (long)z * sz + (long)y * sy + x - org; }
+    __device__ __forceinline__ long at(int y, int x) const { return (long)y * sy + x - org; }
+};
+
+// Point i of an op's index space: LDS levels enumerate power-of-two extents
+// with shifts (points past the last row / plane are skipped by the caller),
+// global levels the exact extents with divisions.
+struct Ix2 {
+    int hi, lo;
+};
+template <bool POW2>
+__device__ __forceinline__ Ix2 split2(int i, int nlo, int llo)
+{
+    if constexpr (POW2) return {i >> llo, i & (nlo - 1)};
+    else return {i / nlo, i % nlo};
+}
+
+// One op over points t0, t0 + nt, ...; FL / CL: level / level + 1 in LDS.
+template <typename T, int dim, bool FL, bool CL>
+__device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __restrict__ lv, T* sm, int b,
+                                          int t0, int nt)
+{
+    const LvPtr<T>& LF = lv[op.level];
+    const Geo g = LF.g;
+    const LvView<T, FL> F(LF, sm, (dim == 2 && !FL) ? (long)b * g.sz : 0, dim);
+    // extents of the loops: exact on global levels, padded to powers of two on LDS ones
+    auto ext = [&](int m) { return FL ? m : m - 1; };
+    auto lg = [](int v) { return v > 0 ? __builtin_ctz((unsigned)v) : 0; };
+    if (op.kind == CO_RB) {
+        const int xh = g.mx / 2, ny = ext(g.my), lxh = lg(xh), lny = lg(ny);
+        if constexpr (dim == 3) {
+            const int cnt = xh * ny * ext(g.mz);
+            for (int i = t0; i < cnt; i += nt) {
+                const Ix2 zr = split2<FL>(i, xh * ny, lxh + lny);
+                const Ix2 yx = split2<FL>(zr.lo, xh, lxh);
+                const int z = 2 + zr.hi, y = 2 + yx.hi;
+                const int x = 2 + ((op.a + y + z) & 1) + 2 * yx.lo;
+                if (x > g.mx || y > g.my || z > g.mz) continue;
+                rb3_point(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, F.at(z, y, x), F.sy, F.sz);
+            }
+        } else {
+            const int cnt = xh * ny;
+            for (int i = t0; i < cnt; i += nt) {
+                const Ix2 yx = split2<FL>(i, xh, lxh);
+                const int y = 2 + yx.hi;
+                const int x = 2 + ((op.a + y) & 1) + 2 * yx.lo;
+                if (x > g.mx || y > g.my) continue;
+                rb2_point(F.phi, F.rhs, F.cx, F.cy, F.cc, F.at(y, x), F.sy);
+            }
+        }
+    } else if (op.kind == CO_RB_DIAG) {
+        const int xh = g.mx / 2, yh = g.my / 2, lxh = lg(xh);
+        for (int i = t0; i < xh * yh; i += nt) {
+            const Ix2 yx = split2<FL>(i, xh, lxh);
+            const int y = (op.b ? 2 : 3) + 2 * yx.hi;
+            const int x = (op.a ? 2 : 3) + 2 * yx.lo;
+            if (y > g.my || x > g.mx) continue;
+            rb2diag_point(F.phi, F.rhs, F.cx, F.cy, F.cc, F.drr, F.dlr, F.at(y, x), F.sy);
+        }
+    } else if (op.kind == CO_RESTRICT) {
+        const LvPtr<T>& LC = lv[op.level + 1];
+        const Geo gc = LC.g;
+        const LvView<T, CL> C(LC, sm, (dim == 2 && !CL) ? (long)b * gc.sz : 0, dim);
+        const int nx = ext(gc.mx), ny = ext(gc.my), lnx = lg(nx), lny = lg(ny);
+        if constexpr (dim == 3) {
+            const int cnt = nx * ny * ext(gc.mz);
+            for (int i = t0; i < cnt; i += nt) {
+                const Ix2 zr = split2<FL>(i, nx * ny, lnx + lny);
+                const Ix2 yx = split2<FL>(zr.lo, nx, lnx);
+                const int I = 2 + zr.hi, J = 2 + yx.hi, K = 2 + yx.lo;
+                if (K > gc.mx || J > gc.my || I > gc.mz) continue;
+                C.rhs[C.at(I, J, K)] =
+                    restrict3_at(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, F.at(2 * I - 1, 2 * J - 1, 2 * K - 1), F.sy,
+                                 F.sz);
+            }
+        } else {
+            for (int i = t0; i < nx * ny; i += nt) {
+                const Ix2 yx = split2<FL>(i, nx, lnx);
+                const int J = 2 + yx.hi, K = 2 + yx.lo;
+                if (K > gc.mx || J > gc.my) continue;
+                const long c = F.at(2 * J - 1, 2 * K - 1);
+                const T v = F.od ? restrict2_at(F.phi, F.rhs, F.cx, F.cy, F.cc, F.drr, F.dlr, c, F.sy)
+                                 : restrict2_at(F.phi, F.rhs, F.cx, F.cy, F.cc, (const T*)nullptr,
+                                                (const T*)nullptr, c, F.sy);
+                C.rhs[C.at(J, K)] = v;
+            }
+        }
+        if (op.b)
+            for (int i = t0; i < C.zn; i += nt) C.phi[i] = (T)0;
+    } else if (op.kind == CO_ZERO) {
+        for (int i = t0; i < F.zn; i += nt) F.phi[i] = (T)0;
+    } else {  // CO_PROLONG
+        const LvPtr<T>& LC = lv[op.level + 1];
+        const Geo gc = LC.g;
+        const LvView<T, CL> C(LC, sm, (dim == 2 && !CL) ? (long)b * gc.sz : 0, dim);
+        const int nx = ext(g.mx), ny = ext(g.my), lnx = lg(nx), lny = lg(ny);
+        if constexpr (dim == 3) {
+            const int cnt = nx * ny * ext(g.mz);
+            for (int i = t0; i < cnt; i += nt) {
+                const Ix2 zr = split2<FL>(i, nx * ny, lnx + lny);
+                const Ix2 yx = split2<FL>(zr.lo, nx, lnx);
+                const int z = 2 + zr.hi, y = 2 + yx.hi, x = 2 + yx.lo;
+                if (x > g.mx || y > g.my || z > g.mz) continue;
+                T corr;
+                if (!prolong3_corr_at<T>([&](int I, int J, int K) -> T { return C.phi[C.at(I, J, K)]; }, x, y, z,
+                                         op.a, corr))
+                    continue;
+                const long p = F.at(z, y, x);
+                F.phi[p] = F.phi[p] + corr;
+            }
+        } else {
+            for (int i = t0; i < nx * ny; i += nt) {
+                const Ix2 yx = split2<FL>(i, nx, lnx);
+                const int y = 2 + yx.hi, x = 2 + yx.lo;
+                if (x > g.mx || y > g.my) continue;
+                T corr;
+                if (!prolong2_corr_at<T>([&](int J, int K) -> T { return C.phi[C.at(J, K)]; }, x, y, op.a, corr))
+                    continue;
+                const long p = F.at(y, x);
+                F.phi[p] = F.phi[p] + corr;
+            }
+        }
+    }
+}
+
+// n elements global -> LDS (or back), U loads in flight per thread
+template <typename T, int NT, typename D, typename S>
+__device__ __forceinline__ void coarse_copy(D dst, S src, int n, int tid)
+{
+    constexpr int U = 8;
+    for (int i0 = 0; i0 < n; i0 += NT * U) {
+        T v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int i = i0 + k * NT + tid;
+            v[k] = src(i < n ? i : n - 1);
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int i = i0 + k * NT + tid;
+            if (i < n) dst(i, v[k]);
+        }
+    }
+}
+
+// ops: nops packed op words, copied to LDS after the levels (element offset
+// ops_at of T); img: this launch's coefficient image (img_n elements per
+// problem, problem stride img_bs), copied to LDS offset 0; levels
+// [lds_from, lds_to) are in LDS; wb_level (-1: none) is the start level when
+// it is staged.
+template <typename T, int NT, int dim>
+__global__ void __launch_bounds__(NT)
+k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops, int nops, int ops_at,
+              const int* __restrict__ act, int lds_from, int lds_to, int wb_level, const T* __restrict__ img,
+              int img_n, long img_bs)
 {
     const int b = blockIdx.x;               // 2D: problem of the batch; 3D: 0
     if (act != nullptr && act[b] == 0) return;
     const int tid = threadIdx.x;
     T* sm = reinterpret_cast<T*>(jsv_dyn_smem);
-    // without staging every pointer stays a global one (no generic accesses)
-    auto staged = [&](int l) { return STAGED && l >= lds_from && l < lds_to; };
-    auto view = [&](int l) -> LvPtr<T> {
-        LvPtr<T> L = lv[l];
-        if (staged(l)) {
-            T* a[8];
-            for (int j = 0; j < 8; ++j) a[j] = L.lds[j] >= 0 ? sm + L.lds[j] : nullptr;
-            L.phi = a[0];
-            L.rhs = a[1];
-            L.cx = a[2];
-            L.cy = a[3];
-            L.cz = a[4];
-            L.cc = a[5];
-            L.drr = a[6];
-            L.dlr = a[7];
-        }
-        return L;
-    };
-    // problem offset of a level's arrays: none in LDS or in 3D
-    auto off = [&](int l, const Geo& g) -> long { return (dim == 2 && !staged(l)) ? (long)b * g.sz : 0; };
-    for (int l = lds_from; STAGED && l < lds_to; ++l) {
-        const LvPtr<T> G = lv[l];
-        const long np = dim == 3 ? G.g.n : G.g.sz, gb = dim == 2 ? (long)b * G.g.sz : 0;
-        const T* src[8] = {nullptr, G.rhs, G.cx, G.cy, G.cz, G.cc, G.drr, G.dlr};
-        for (int j = 1; j < 8; ++j) {
-            if (G.lds[j] < 0 || src[j] == nullptr) continue;
-            T* d = sm + G.lds[j];
-            for (long i = tid; i < np; i += COARSE_THREADS) d[i] = src[j][gb + i];
-        }
+    unsigned* sops = reinterpret_cast<unsigned*>(sm + ops_at);
+    coarse_copy<unsigned, NT>([&](int i, unsigned v) { sops[i] = v; }, [&](int i) { return ops[i]; }, nops, tid);
+    if (img_n > 0) {
+        const T* src = img + (long)b * img_bs;
+        coarse_copy<T, NT>([&](int i, T v) { sm[i] = v; }, [&](int i) { return src[i]; }, img_n, tid);
     }
-    if (STAGED) __syncthreads();
-    for (int o = 0; o < nops; ++o) {
-        const CoarseOp op = ops[o];
-        const LvPtr<T> L = view(op.level);
-        const Geo& g = L.g;
-        const long base = off(op.level, g);
-        if (op.kind == CO_RB) {
-            const int xh = g.mx / 2, ny = g.my - 1;
-            if (dim == 3) {
-                const int cnt = xh * ny * (g.mz - 1);
-                for (int i = tid; i < cnt; i += COARSE_THREADS) {
-                    const int z = 2 + i / (xh * ny), r = i % (xh * ny);
-                    const int y = 2 + r / xh;
-                    const int x = 2 + ((op.a + y + z) & 1) + 2 * (r % xh);
-                    if (x > g.mx) continue;
-                    rb3_point(L.phi, L.rhs, L.cx, L.cy, L.cz, L.cc,
-                              (long)z * g.sz + (long)y * g.sy + x, g.sy, g.sz);
-                }
-            } else {
-                const int cnt = xh * ny;
-                for (int i = tid; i < cnt; i += COARSE_THREADS) {
-                    const int y = 2 + i / xh;
-                    const int x = 2 + ((op.a + y) & 1) + 2 * (i % xh);
-                    if (x > g.mx) continue;
-                    rb2_point(L.phi, L.rhs, L.cx, L.cy, L.cc, base + (long)y * g.sy + x, g.sy);
-                }
-            }
-        } else if (op.kind == CO_RB_DIAG) {
-            const int xh = g.mx / 2, yh = g.my / 2;
-            for (int i = tid; i < xh * yh; i += COARSE_THREADS) {
-                const int y = (op.b ? 2 : 3) + 2 * (i / xh);
-                const int x = (op.a ? 2 : 3) + 2 * (i % xh);
-                if (y > g.my || x > g.mx) continue;
-                rb2diag_point(L.phi, L.rhs, L.cx, L.cy, L.cc, L.drr, L.dlr,
-                              base + (long)y * g.sy + x, g.sy);
-            }
-        } else if (op.kind == CO_RESTRICT) {
-            const LvPtr<T> C = view(op.level + 1);
-            const Geo& gc = C.g;
-            const int nx = gc.mx - 1, ny = gc.my - 1;
-            if (dim == 3) {
-                const int cnt = nx * ny * (gc.mz - 1);
-                for (int i = tid; i < cnt; i += COARSE_THREADS) {
-                    const int I = 2 + i / (nx * ny), r = i % (nx * ny);
-                    const int J = 2 + r / nx, K = 2 + r % nx;
-                    const long c = (long)(2 * I - 1) * g.sz + (long)(2 * J - 1) * g.sy + (2 * K - 1);
-                    C.rhs[(long)I * gc.sz + (long)J * gc.sy + K] =
-                        restrict3_at(L.phi, L.rhs, L.cx, L.cy, L.cz, L.cc, c, g.sy, g.sz);
-                }
-            } else {
-                const long cbase = off(op.level + 1, gc);
-                for (int i = tid; i < nx * ny; i += COARSE_THREADS) {
-                    const int J = 2 + i / nx, K = 2 + i % nx;
-                    const long c = base + (long)(2 * J - 1) * g.sy + (2 * K - 1);
-                    C.rhs[cbase + (long)J * gc.sy + K] =
-                        restrict2_at(L.phi, L.rhs, L.cx, L.cy, L.cc, L.drr, L.dlr, c, g.sy);
+    if (wb_level >= 0) {
+        // the start level's rhs over the image's zeros (other threads wrote those)
+        __syncthreads();
+        // compact point i is padded point (z, y, x) + 1
+        const LvPtr<T>& W = lv[wb_level];
+        const T* src = W.rhs + (dim == 2 ? (long)b * W.g.sz : 0);
+        T* d = sm + W.lds[1];
+        coarse_copy<T, NT>([&](int i, T v) { d[i] = v; },
+                           [&](int i) {
+                               const int z = i / W.lsz, r = i - z * W.lsz, y = r / W.lsy, x = r - y * W.lsy;
+                               return src[(dim == 3 ? (long)(z + 1) * W.g.sz : 0) + (long)(y + 1) * W.g.sy + x + 1];
+                           },
+                           W.ln, tid);
+    }
+    __syncthreads();
+    for (int o = 0; o < nops;) {
+        const CoarseOp op = coarse_unpack(__builtin_amdgcn_readfirstlane(sops[o]));
+        if (op.run > 0) {
+            if (tid < COARSE_WAVE) {
+                for (int k = 0; k < op.run; ++k) {
+                    const CoarseOp q = k == 0 ? op : coarse_unpack(__builtin_amdgcn_readfirstlane(sops[o + k]));
+                    coarse_op<T, dim, true, true>(q, lv, sm, b, tid, COARSE_WAVE);
+                    // the next op reads what other lanes of this wave wrote: no
+                    // reordering of LDS accesses across this point
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
-        } else if (op.kind == CO_ZERO) {
-            const long cnt = dim == 2 ? g.sz : g.n;
-            for (long i = tid; i < cnt; i += COARSE_THREADS) L.phi[base + i] = (T)0;
-        } else {  // CO_PROLONG
-            const LvPtr<T> C = view(op.level + 1);
-            const Geo& gc = C.g;
-            const int nx = g.mx - 1, ny = g.my - 1;
-            if (dim == 3) {
-                const int cnt = nx * ny * (g.mz - 1);
-                for (int i = tid; i < cnt; i += COARSE_THREADS) {
-                    const int z = 2 + i / (nx * ny), r = i % (nx * ny);
-                    const int y = 2 + r / nx, x = 2 + r % nx;
-                    T corr;
-                    if (!prolong3_corr(C.phi, gc.sy, gc.sz, x, y, z, op.a, corr)) continue;
-                    const long p = (long)z * g.sz + (long)y * g.sy + x;
-                    L.phi[p] = L.phi[p] + corr;
-                }
-            } else {
-                const long cbase = off(op.level + 1, gc);
-                for (int i = tid; i < nx * ny; i += COARSE_THREADS) {
-                    const int y = 2 + i / nx, x = 2 + i % nx;
-                    T corr;
-                    if (!prolong2_corr(C.phi + cbase, gc.sy, x, y, op.a, corr)) continue;
-                    const long p = base + (long)y * g.sy + x;
-                    L.phi[p] = L.phi[p] + corr;
-                }
-            }
+            o += op.run;
+        } else {
+            if (op.level >= lds_from && op.level < lds_to)
+                coarse_op<T, dim, true, true>(op, lv, sm, b, tid, NT);
+            else if (op.level + 1 >= lds_from && op.level + 1 < lds_to)
+                coarse_op<T, dim, false, true>(op, lv, sm, b, tid, NT);
+            else
+                coarse_op<T, dim, false, false>(op, lv, sm, b, tid, NT);
+            ++o;
         }
         __syncthreads();
     }
-    if (STAGED && wb_level >= 0 && staged(wb_level)) {
-        // the staged top level's phi goes back for the prolongation after the program
-        const LvPtr<T> G = lv[wb_level];
-        const long np = dim == 3 ? G.g.n : G.g.sz, gb = dim == 2 ? (long)b * G.g.sz : 0;
-        const T* d = sm + G.lds[0];
-        for (long i = tid; i < np; i += COARSE_THREADS) G.phi[gb + i] = d[i];
+    if (wb_level >= 0) {
+        // the staged start level's phi goes back, whole padded array (ghost layers 0)
+        const LvPtr<T>& W = lv[wb_level];
+        const int np = (int)(dim == 3 ? W.g.n : W.g.sz);
+        T* dst = W.phi + (dim == 2 ? (long)b * W.g.sz : 0);
+        const T* s = sm + W.lds[0];
+        const int nx = W.g.nx, ny = W.g.ny;
+        coarse_copy<T, NT>([&](int i, T v) { dst[i] = v; },
+                           [&](int i) {
+                               const int z = dim == 3 ? i / (nx * ny) : 1, r = dim == 3 ? i - (z * nx * ny) : i;
+                               const int y = r / nx, x = r - y * nx;
+                               const bool in = z >= 1 && y >= 1 && x >= 1 && y <= ny - 2 && x <= nx - 2 &&
+                                               (dim == 2 || z <= W.g.nz - 2);
+                               return in ? s[(dim == 3 ? (z - 1) * W.lsz : 0) + (y - 1) * W.lsy + x - 1] : (T)0;
+                           },
+                           np, tid);
     }
+}
+
+// Image of the staged levels (the LDS region in its layout): one array of
+// level geometry g gathered into its compact copy at offset dst of every
+// problem's image (stride dbs); src has problem stride sbs.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_stage_image(T* __restrict__ img, long dbs, int dst, const T* __restrict__ src, long sbs, Geo g, int lsy,
+              int lsz, int ln, int dim)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ln) return;
+    const long b = blockIdx.y;
+    const int z = i / lsz, r = i - z * lsz, y = r / lsy, x = r - y * lsy;
+    const long q = (dim == 3 ? (long)(z + 1) * g.sz : 0) + (long)(y + 1) * g.sy + x + 1;
+    img[b * dbs + dst + i] = src[b * sbs + q];
 }
 
 // ---------------------------------------------------------------------------

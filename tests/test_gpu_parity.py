@@ -280,6 +280,59 @@ def test_cycle_coarse_program_bitexact(case, cyc, coarse_m, lds):
         np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"call {rep}")
 
 
+@pytest.mark.parametrize("solo,threads", [(0, 1024), (64, 256), (64, 512), (16, 1024)])
+@pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4]], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", COARSE_CASES, ids=[c[0] for c in COARSE_CASES])
+def test_cycle_coarse_program_launch_shapes(case, cyc, solo, threads):
+    """The coarse program from level 1 down with its small ops run by one wave
+    (or never) and other workgroup sizes: still the oracle's bits."""
+    from jsolver_amd import _lib
+    lib = _lib.load()
+    dp, op, phi, rhs = _setup(case, seed=19)
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, 1 << 20))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_SOLO, solo))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_THREADS, threads))
+    kw = cyc[1]
+    c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+    want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
+    for rep in range(2):
+        dp.upload(0, 0, phi)
+        dp.upload(0, 1, rhs)
+        dp.cycle(cfg(**kw))
+        np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"call {rep}")
+
+
+def test_coarse_tuning_limits():
+    from jsolver_amd import _lib
+    lib = _lib.load()
+    dp, _, _, _ = _setup(CASES[0], seed=1)
+    assert lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_SOLO, 65) != 0
+    assert lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_THREADS, 128) != 0
+
+
+@pytest.mark.parametrize("dim,ms", [(2, (64, 64)), (3, (16, 16, 16))])
+def test_cycle_coarse_program_new_operator(dim, ms):
+    """Same plan, a second operator of the same kind: the coarse program's
+    LDS coefficient image must follow the new coefficients."""
+    dp = None
+    for seed in (31, 32):
+        phi, rhs, lam, D, _ = rand_problem(dim, ms, seed=seed)
+        if dp is None:
+            dp = DevPlan(dim, ms, lam, D)
+        else:
+            dev = dp.plan.dev
+            t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+            dp.plan.set_operator(t(lam), [t(d) for d in D])
+        op = oracle_operator(dim, ms, lam, D)
+        for kw in (dict(), dict(gamma=2)):
+            c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+            dp.upload(0, 0, phi)
+            dp.upload(0, 1, rhs)
+            dp.cycle(cfg(**kw))
+            np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle),
+                                          err_msg=f"seed {seed} {kw}")
+
+
 def test_cycle_coarse_program_operator_switch():
     """One plan, operator switched D_xy -> 5-point -> D_xy: each cycle still
     gives the oracle's bits (the D_xy arrays stay allocated in between)."""
