@@ -605,6 +605,24 @@ const jsv_plan::CoarseProg* find_prog(const jsv_plan* P, const jsv_config& c, bo
     return nullptr;
 }
 
+// A smoothing op with no point of its colour on the level (restriction and
+// prolongation always have points).
+bool op_empty(const jsv_plan* P, const std::vector<Geo>& G, const jsv::CoarseOp& op)
+{
+    const Geo& g = G[op.level];
+    if (op.kind == jsv::CO_RB_DIAG) {
+        // x = (a ? 2 : 3) + 2i <= mx, y = (b ? 2 : 3) + 2j <= my
+        const int nx = op.a ? g.mx / 2 : (g.mx - 1) / 2, ny = op.b ? g.my / 2 : (g.my - 1) / 2;
+        return nx <= 0 || ny <= 0;
+    }
+    if (op.kind != jsv::CO_RB) return false;
+    // x = 2 + ((a + y [+ z]) & 1) + 2i <= mx for some y (and z) in [2, m]
+    for (int z = 2; z <= (P->dim == 3 ? g.mz : 2); ++z)
+        for (int y = 2; y <= g.my; ++y)
+            if (2 + ((op.a + y + (P->dim == 3 ? z : 0)) & 1) <= g.mx) return false;
+    return true;
+}
+
 // Loop count of one op in k_coarse_prog, for the one-wave runs: exact
 // extents on global levels, power-of-two extents on staged ones (whose arrays
 // are compact, ln elements).
@@ -687,7 +705,8 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
                 loff[l][j] = o;
                 o += ln[l];
             }
-    const int img_n = o;
+    const int img_n = (int)((o * sizeof(T) + 15) / 16 * 16 / sizeof(T));   // whole 16-byte pieces
+    used = (size_t)img_n * sizeof(T);
     P->clds_from[pi] = lds_from;
     P->clds_to[pi] = lds_to;
     P->clds_bytes[pi] = used;
@@ -743,6 +762,35 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
             P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
                                 P->offdiag ? 1 : 0, lds_from, P->coarse_solo, nullptr, (int)ops.size()});
             continue;
+        }
+        // Ops that change nothing go: smoothing halves without points, and on
+        // a level whose only unknown neighbours boundary nodes every
+        // smoothing op after the first of a visit (the unknown's update reads
+        // only rhs and coefficients, so it recomputes the same bits).  The
+        // first op zeroes a staged start level's phi, which the image
+        // already holds as zeros.
+        auto single = [&](int l) {
+            const Geo& g = H.g[l];
+            return g.mx == 2 && g.my == 2 && (P->dim == 2 || g.mz == 2);
+        };
+        {
+            std::vector<jsv::CoarseOp> kept;
+            bool done_single = false;   // the single unknown was updated in this visit
+            for (const jsv::CoarseOp& op : ops) {
+                const bool smooth = op.kind == jsv::CO_RB || op.kind == jsv::CO_RB_DIAG;
+                if (smooth && op_empty(P, H.g, op)) continue;
+                if (smooth && single(op.level)) {
+                    if (done_single) continue;
+                    done_single = true;
+                } else if (!(smooth && single(op.level))) {
+                    done_single = false;
+                }
+                if (op.kind == jsv::CO_ZERO && op.level == start && start >= lds_from && start < lds_to &&
+                    kept.empty())
+                    continue;
+                kept.push_back(op);
+            }
+            ops.swap(kept);
         }
         // runs of small ops on staged levels go to one wave
         auto staged = [&](int l) { return l >= lds_from && l < lds_to; };

@@ -1237,8 +1237,9 @@ __device__ __forceinline__ void coarse_copy(D dst, S src, int n, int tid)
 }
 
 // ops: nops packed op words, copied to LDS after the levels (element offset
-// ops_at of T); img: this launch's coefficient image (img_n elements per
-// problem, problem stride img_bs), copied to LDS offset 0; levels
+// ops_at of T); img: this launch's image of the staged levels (img_n
+// elements per problem, a multiple of 16 bytes; problem stride img_bs),
+// copied to LDS offset 0; levels
 // [lds_from, lds_to) are in LDS; wb_level (-1: none) is the start level when
 // it is staged.
 template <typename T, int NT, int dim>
@@ -1254,8 +1255,11 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops,
     unsigned* sops = reinterpret_cast<unsigned*>(sm + ops_at);
     coarse_copy<unsigned, NT>([&](int i, unsigned v) { sops[i] = v; }, [&](int i) { return ops[i]; }, nops, tid);
     if (img_n > 0) {
-        const T* src = img + (long)b * img_bs;
-        coarse_copy<T, NT>([&](int i, T v) { sm[i] = v; }, [&](int i) { return src[i]; }, img_n, tid);
+        // 16-byte pieces (img_n is a multiple of 16 bytes)
+        const uint4* src = reinterpret_cast<const uint4*>(img + (long)b * img_bs);
+        uint4* dst = reinterpret_cast<uint4*>(sm);
+        coarse_copy<uint4, NT>([&](int i, uint4 v) { dst[i] = v; }, [&](int i) { return src[i]; },
+                               (int)(img_n * sizeof(T) / 16), tid);
     }
     if (wb_level >= 0) {
         // the start level's rhs over the image's zeros (other threads wrote those)
