@@ -94,7 +94,8 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         smoothing schedule (default 0 = whole level per launch)
  *   JSV_TUNE_TWO_PASS_MIN level size (elements) from which the restriction
  *                         stores first-half residuals and gathers them in a
- *                         second pass (default 2^20)
+ *                         second pass (default 2^20; 2^13 for 3D FP32 levels;
+ *                         the tuned value applies to both precisions)
  *   JSV_TUNE_FUSED_MIN_M  3D levels with mz >= this run each RBGS sweep as one
  *                         fused out-of-place z-marching kernel (default 128;
  *                         0 = never)

@@ -112,7 +112,7 @@ struct jsv_plan {
     int* d_actk = nullptr;          // per-cycle activity rows of a cycle block (ACTK_ROWS x batch)
     int* h_actk = nullptr;          // pinned staging of d_actk
     long slab_bytes = 0;            // slab size of the 3D two-launch smoothing schedule (0: whole level)
-    long two_pass_min = 1L << 20;   // level size from which restriction runs in two passes
+    long two_pass_min[2] = {1L << 20, 1L << 20};   // FP64 / FP32: level size from which restriction runs in two passes
     int fused_min_m = 128;          // 3D levels with mz >= this use the fused out-of-place sweep
     int zchunk = 0;                 // planes per workgroup of the fused sweep (0: by level size)
     int sweep_rows = 0;             // rows per workgroup tile of the fused sweep (0: by precision)
@@ -200,11 +200,11 @@ int alloc_hier(jsv_plan* P, Hier<T>& H, bool with_coeffs)
     RETCHK(one(H.cc, with_coeffs));
     RETCHK(one(H.drr, false));
     RETCHK(one(H.dlr, false));
-    // two-pass restriction on levels above ~1M points (one-pass below: there
-    // the stencil re-reads hit in L2 and one launch is cheaper)
+    // two-pass restriction on large levels (one-pass below: there the
+    // stencil re-reads hit in L2 and one launch is cheaper)
     H.res.assign(L, nullptr);
     for (int l = 0; l + 1 < L; ++l)
-        if (H.g[l].n >= P->two_pass_min)
+        if (H.g[l].n >= P->two_pass_min[sizeof(T) == 4 ? 1 : 0])
             RETCHK(dalloc(P, (void**)&H.res[l], sizeof(T) * H.g[l].n));
     H.phi2.assign(L, nullptr);
     if (P->dim == 3)
@@ -1073,6 +1073,8 @@ int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double i
     P->coarse_m[0] = dim == 3 ? 8 : 64;
     P->coarse_m[1] = dim == 3 ? 16 : 64;
     P->coarse_lds = 1;
+    // 3D FP32 restrictions of the 64^3 and 32^3 levels are faster in two passes
+    P->two_pass_min[1] = dim == 3 ? (1L << 13) : (1L << 20);
     {
         int dev = 0, ncu = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -1163,10 +1165,10 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         return 0;
     }
     if (key == JSV_TUNE_TWO_PASS_MIN) {
-        P->two_pass_min = (long)value;
+        P->two_pass_min[0] = P->two_pass_min[1] = (long)value;
         auto realloc_res = [&](auto& H) -> int {
             for (int l = 0; l + 1 < P->levels && l < (int)H.res.size(); ++l)
-                if (H.res[l] == nullptr && H.g[l].n >= P->two_pass_min)
+                if (H.res[l] == nullptr && H.g[l].n >= value)
                     RETCHK(dalloc(P, (void**)&H.res[l], sizeof(*H.phi[0]) * H.g[l].n));
             return 0;
         };
@@ -1174,7 +1176,7 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         if (P->prec == JSV_MIXED) RETCHK(realloc_res(P->h32));
         // levels below the threshold go back to one pass
         for (int l = 0; l + 1 < P->levels; ++l) {
-            if (P->h64.g[l].n < P->two_pass_min) {
+            if (P->h64.g[l].n < value) {
                 if (l < (int)P->h64.res.size()) P->h64.res[l] = nullptr;
                 if (l < (int)P->h32.res.size()) P->h32.res[l] = nullptr;
             }
