@@ -877,10 +877,14 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
     return 0;
 }
 
+void adj_drop_graphs(const jsv_plan* P);
+void adj_forget(const jsv_plan* P);
+
 void drop_graphs(jsv_plan* P)
 {
     for (auto& gr : P->graphs) (void)hipGraphExecDestroy(gr.exec);
     P->graphs.clear();
+    adj_drop_graphs(P);
 }
 
 // One top-level cycle.  The kernel sequence of a cycle depends only on the
@@ -1123,6 +1127,7 @@ int jsv_plan_create(int dim, int mx, int my, int mz, double idel_x, double idel_
 int jsv_plan_destroy(jsv_plan* P)
 {
     if (P == nullptr) return 0;
+    adj_forget(P);   // the adjoint state holds this plan's buffers
     for (auto& gr : P->graphs) (void)hipGraphExecDestroy(gr.exec);
     if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
     for (void* p : P->allocs) (void)hipFree(p);
@@ -1882,6 +1887,14 @@ struct Adj {
     std::vector<std::vector<double*>> pool;   // per level snapshot slots
     std::vector<size_t> used;
     std::vector<std::pair<int, double*>> tape;
+    // one reversed cycle (recompute with tape + transposed cycle) per
+    // configuration, captured as a hipGraph: every buffer it touches belongs
+    // to the plan, and a cycle's input arrives in phi of level 0
+    struct Graph {
+        int gamma, fcycle, skip_pre, nu1, nu2;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
 };
 
 // snapshots per level for one cycle (mirrors mg_cycle)
@@ -1947,8 +1960,9 @@ int snap(jsv_plan* P, Adj& A, int l, hipStream_t s)
 {
     if (A.used[l] >= A.pool[l].size()) return fail("adjoint tape overflow at level %d", l);
     double* dst = A.pool[l][A.used[l]++];
-    HIPCHK(hipMemcpyAsync(dst, P->h64.phi[l], sizeof(double) * P->h64.g[l].n,
-                          hipMemcpyDeviceToDevice, s));
+    // a copy kernel, not a memcpy node, so the reversed cycle can be captured
+    const long n = P->h64.g[l].n;
+    hipLaunchKernelGGL((jsv::k_cast<double, double>), grid_flat(n), dim3(256), 0, s, P->h64.phi[l], dst, n);
     A.tape.push_back({l, dst});
     return 0;
 }
@@ -2080,6 +2094,64 @@ Adj& adj_of(jsv_plan* P);
 namespace {
 std::map<const jsv_plan*, Adj> g_adj;
 Adj& adj_of(jsv_plan* P) { return g_adj[P]; }
+
+void adj_drop_graphs(const jsv_plan* P)
+{
+    auto it = g_adj.find(P);
+    if (it == g_adj.end()) return;
+    for (auto& gr : it->second.graphs) (void)hipGraphExecDestroy(gr.exec);
+    it->second.graphs.clear();
+}
+
+// One reversed cycle: phi of level 0 holds the cycle's input.  Replayed from
+// a captured graph (see run_cycle) unless graphs are off.
+int adj_run_cycle(jsv_plan* P, Adj& A, const jsv_config& c, hipStream_t s)
+{
+    const int L = P->levels;
+    auto eager = [&](hipStream_t st) -> int {
+        A.used.assign(L, 0);
+        A.tape.clear();
+        RETCHK(rec_cycle(P, A, c, 0, c.fcycle != 0, c.skip_pre != 0, st));
+        RETCHK(adj_cycle(P, A, c, 0, c.fcycle != 0, c.skip_pre != 0, st));
+        if (!A.tape.empty()) return fail("adjoint tape not consumed (%zu left)", A.tape.size());
+        return zero_rim_any(P->h64.g[0], P->dim, A.g[0], st);   // cycle input passes do_BCs
+    };
+    if (!P->use_graphs) return eager(s);
+    for (auto& gr : A.graphs)
+        if (gr.gamma == c.gamma && gr.fcycle == (c.fcycle != 0) && gr.skip_pre == (c.skip_pre != 0) &&
+            gr.nu1 == c.nu1 && gr.nu2 == c.nu2) {
+            HIPCHK(hipGraphLaunch(gr.exec, s));
+            return 0;
+        }
+    if (P->cap_stream == nullptr) HIPCHK(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, s));
+    HIPCHK(hipStreamWaitEvent(P->cap_stream, ev, 0));
+    (void)hipEventDestroy(ev);
+    HIPCHK(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = eager(P->cap_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(P->cap_stream, &graph);
+    if (rc != 0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail("adjoint graph capture failed: %s", hipGetErrorString(e));
+    Adj::Graph gr{c.gamma, c.fcycle != 0 ? 1 : 0, c.skip_pre != 0 ? 1 : 0, c.nu1, c.nu2, nullptr};
+    const hipError_t ei = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail("adjoint graph instantiate failed: %s", hipGetErrorString(ei));
+    A.graphs.push_back(gr);
+    HIPCHK(hipGraphLaunch(gr.exec, s));
+    return 0;
+}
+
+void adj_forget(const jsv_plan* P)
+{
+    adj_drop_graphs(P);
+    g_adj.erase(P);
+}
 }  // namespace
 
 extern "C" {
@@ -2142,14 +2214,9 @@ int jsv_adjoint(jsv_plan* P, const double* phi0, const double* rhs, const double
     RETCHK(zero_rim_any(H.g[0], P->dim, A.g[0], s));
     for (int64_t k = c->max_iters; k >= 1; --k) {
         // recompute cycle k from its input with a tape, then reverse it
-        A.used.assign(L, 0);
-        A.tape.clear();
         HIPCHK(hipMemcpyAsync(H.phi[0], ckpt + (k - 1) * n, sizeof(double) * n,
                               hipMemcpyDeviceToDevice, s));
-        RETCHK(rec_cycle(P, A, *c, 0, c->fcycle != 0, c->skip_pre != 0, s));
-        RETCHK(adj_cycle(P, A, *c, 0, c->fcycle != 0, c->skip_pre != 0, s));
-        if (!A.tape.empty()) return fail("adjoint tape not consumed (%zu left)", A.tape.size());
-        RETCHK(zero_rim_any(H.g[0], P->dim, A.g[0], s));   // cycle input passes do_BCs
+        RETCHK(adj_run_cycle(P, A, *c, s));
         LAUNCHCHK();
     }
     if (c->skip_pre && c->nu1 > 0) {
