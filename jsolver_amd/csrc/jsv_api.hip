@@ -698,15 +698,23 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
     }
     std::vector<std::array<int, 8>> loff(P->levels);
     for (auto& a : loff) a.fill(-1);
+    // the start level's rhs goes last, after the image: the kernel copies it
+    // in from the restriction's output in the same pass as the image
+    const bool start_staged = start >= lds_from && start < lds_to;
     int o = 0;
     for (int l = lds_from; l < lds_to; ++l)
         for (int j = 0; j < 8; ++j)
-            if (has(j)) {
+            if (has(j) && !(start_staged && l == start && j == 1)) {
                 loff[l][j] = o;
                 o += ln[l];
             }
     const int img_n = (int)((o * sizeof(T) + 15) / 16 * 16 / sizeof(T));   // whole 16-byte pieces
-    used = (size_t)img_n * sizeof(T);
+    o = img_n;
+    if (start_staged) {
+        loff[start][1] = o;
+        o += ln[start];
+    }
+    used = (size_t)o * sizeof(T);
     P->clds_from[pi] = lds_from;
     P->clds_to[pi] = lds_to;
     P->clds_bytes[pi] = used;
@@ -842,8 +850,8 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
         const int lf = P->clds_from[pi], lt = P->clds_to[pi];
         const dim3 grid((unsigned)(P->dim == 2 ? P->batch : 1));
         const auto* lvp = (const jsv::LvPtr<T>*)P->lv_dev[pi];
-        // LDS: the staged levels (the image), then the op words
-        const int ops_at = P->img_n[pi];
+        // LDS: the staged levels (the image, then the start level's rhs), then the op words
+        const int ops_at = (int)(P->clds_bytes[pi] / sizeof(T));
         const size_t lds = P->clds_bytes[pi] + sizeof(unsigned) * (size_t)pr->nops;
         const int wb = (lf <= start && start < lt) ? start : -1;
         const T* img = (const T*)P->img_dev[pi];

@@ -1253,35 +1253,68 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops,
     const int tid = threadIdx.x;
     T* sm = reinterpret_cast<T*>(jsv_dyn_smem);
     unsigned* sops = reinterpret_cast<unsigned*>(sm + ops_at);
-    coarse_copy<unsigned, NT>([&](int i, unsigned v) { sops[i] = v; }, [&](int i) { return ops[i]; }, nops, tid);
-    if (img_n > 0) {
-        // 16-byte pieces (img_n is a multiple of 16 bytes)
-        const uint4* src = reinterpret_cast<const uint4*>(img + (long)b * img_bs);
-        uint4* dst = reinterpret_cast<uint4*>(sm);
-        coarse_copy<uint4, NT>([&](int i, uint4 v) { dst[i] = v; }, [&](int i) { return src[i]; },
-                               (int)(img_n * sizeof(T) / 16), tid);
-    }
-    if (wb_level >= 0) {
-        // the start level's rhs over the image's zeros (other threads wrote those)
-        __syncthreads();
-        // compact point i is padded point (z, y, x) + 1
-        const LvPtr<T>& W = lv[wb_level];
-        const T* src = W.rhs + (dim == 2 ? (long)b * W.g.sz : 0);
-        T* d = sm + W.lds[1];
-        coarse_copy<T, NT>([&](int i, T v) { d[i] = v; },
-                           [&](int i) {
-                               const int z = i / W.lsz, r = i - z * W.lsz, y = r / W.lsy, x = r - y * W.lsy;
-                               return src[(dim == 3 ? (long)(z + 1) * W.g.sz : 0) + (long)(y + 1) * W.g.sy + x + 1];
-                           },
-                           W.ln, tid);
+    {
+        // The image goes global -> LDS directly (16 bytes per lane, no
+        // registers; wave w moves pieces [c, c + 64) to LDS piece c); the op
+        // words and the start level's rhs (after the image: no overlap) go
+        // through registers with all their loads issued before any store.
+        const int n16 = (int)(img_n * sizeof(T) / 16);
+        const unsigned* isrc = reinterpret_cast<const unsigned*>(img + (long)b * img_bs);
+        const int lane = tid & (COARSE_WAVE - 1);
+        for (int c = tid - lane; c < n16; c += NT)
+            if (c + lane < n16)
+                __builtin_amdgcn_global_load_lds(isrc + 4 * (c + lane),
+                                                 (__attribute__((address_space(3))) void*)(sm + c * (16 / sizeof(T))),
+                                                 16, 0, 0);
+        const LvPtr<T>* W = wb_level >= 0 ? &lv[wb_level] : nullptr;
+        const int nr = W != nullptr ? W->ln : 0;
+        const T* rsrc = W != nullptr ? W->rhs + (dim == 2 ? (long)b * W->g.sz : 0) : nullptr;
+        T* rdst = W != nullptr ? sm + W->lds[1] : nullptr;
+        constexpr int UO = 2, UR = 8;
+        auto cdiv = [](int a, int q) { return (a + q - 1) / q; };
+        const int rounds = max(cdiv(nops, NT * UO), cdiv(nr, NT * UR));
+        for (int r = 0; r < rounds; ++r) {
+            unsigned vo[UO];
+            T vr[UR];
+#pragma unroll
+            for (int k = 0; k < UO; ++k) {
+                const int i = (r * UO + k) * NT + tid;
+                vo[k] = ops[i < nops ? i : nops - 1];
+            }
+            if (nr > 0) {
+#pragma unroll
+                for (int k = 0; k < UR; ++k) {
+                    int i = (r * UR + k) * NT + tid;
+                    i = i < nr ? i : nr - 1;
+                    // compact point i is padded point (z, y, x) + 1
+                    const int z = i / W->lsz, q = i - z * W->lsz, y = q / W->lsy, x = q - y * W->lsy;
+                    vr[k] = rsrc[(dim == 3 ? (long)(z + 1) * W->g.sz : 0) + (long)(y + 1) * W->g.sy + x + 1];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < UO; ++k) {
+                const int i = (r * UO + k) * NT + tid;
+                if (i < nops) sops[i] = vo[k];
+            }
+            if (nr > 0) {
+#pragma unroll
+                for (int k = 0; k < UR; ++k) {
+                    const int i = (r * UR + k) * NT + tid;
+                    if (i < nr) rdst[i] = vr[k];
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);   // the direct-to-LDS loads have landed
     }
     __syncthreads();
     for (int o = 0; o < nops;) {
         const CoarseOp op = coarse_unpack(__builtin_amdgcn_readfirstlane(sops[o]));
         if (op.run > 0) {
             if (tid < COARSE_WAVE) {
+                unsigned wn = sops[o + 1 < nops ? o + 1 : o];   // next op word, read ahead
                 for (int k = 0; k < op.run; ++k) {
-                    const CoarseOp q = k == 0 ? op : coarse_unpack(__builtin_amdgcn_readfirstlane(sops[o + k]));
+                    const CoarseOp q = k == 0 ? op : coarse_unpack(__builtin_amdgcn_readfirstlane(wn));
+                    if (k > 0) wn = sops[o + k + 1 < nops ? o + k + 1 : o + k];
                     coarse_op<T, dim, true, true>(q, lv, sm, b, tid, COARSE_WAVE);
                     // the next op reads what other lanes of this wave wrote: no
                     // reordering of LDS accesses across this point
