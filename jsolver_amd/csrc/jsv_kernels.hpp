@@ -861,6 +861,14 @@ k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__
 // reads) once; pass 2 combines them per coarse unknown in the reference's
 // order.  Same bits as the one-pass kernels above, without evaluating each
 // edge residual four times.
+// First-half residuals of a 3D level are stored compactly: point (z, y, x)
+// with x + y + z odd, z, y, x in 2..m, at ((z-2)(my-1) + y-2) * mx/2 + (x-2)/2
+// (x = 2 + parity + 2i, so (x-2)/2 = i): half the bytes of a full array.
+__device__ __forceinline__ long res3_at(const Geo& g, int z, int y, int x)
+{
+    return ((long)(z - 2) * (g.my - 1) + (y - 2)) * (g.mx >> 1) + ((x - 2) >> 1);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
@@ -873,7 +881,7 @@ k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
     const int x = 2 + ((1 + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
     if (x > g.mx) return;
     const long p = (long)z * g.sz + (long)y * g.sy + x;
-    r[p] = resid3<T, T>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);
+    r[res3_at(g, z, y, x)] = resid3<T, T>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);
 }
 
 template <typename T>
@@ -884,20 +892,22 @@ k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, 
     const int J = 2 + blockIdx.y * BY + threadIdx.y;
     const int K = 2 + blockIdx.x * BX + threadIdx.x;
     if (J > gc.my || K > gc.mx) return;
-    const long sy = gf.sy, sz = gf.sz;
-    const long c = (long)(2 * I - 1) * sz + (long)(2 * J - 1) * sy + (2 * K - 1);
-    T e = r[c + sy + 1] + r[c + sy - 1];
-    e = e + r[c - sy + 1];
-    e = e + r[c - sy - 1];
-    e = e + r[c + sz + 1];
-    e = e + r[c + sz - 1];
-    e = e + r[c - sz + 1];
-    e = e + r[c - sz - 1];
-    e = e + r[c + sz + sy];
-    e = e + r[c + sz - sy];
-    e = e + r[c - sz + sy];
-    e = e + r[c - sz - sy];
-    rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * r[c];
+    // residual at fine offset (dz, dy, dx) from the centre (2I-1, 2J-1, 2K-1)
+    auto R = [&](int dz, int dy, int dx) -> T {
+        return r[res3_at(gf, 2 * I - 1 + dz, 2 * J - 1 + dy, 2 * K - 1 + dx)];
+    };
+    T e = R(0, 1, 1) + R(0, 1, -1);
+    e = e + R(0, -1, 1);
+    e = e + R(0, -1, -1);
+    e = e + R(1, 0, 1);
+    e = e + R(1, 0, -1);
+    e = e + R(-1, 0, 1);
+    e = e + R(-1, 0, -1);
+    e = e + R(1, 1, 0);
+    e = e + R(1, -1, 0);
+    e = e + R(-1, 1, 0);
+    e = e + R(-1, -1, 0);
+    rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * R(0, 0, 0);
     if (zphi != nullptr) zero_with_rim(zphi, gc, 0, 3, I, J, K);
 }
 
