@@ -1979,16 +1979,16 @@ int first_parity(const jsv_plan* P) { return P->dim == 3 ? 1 : 0; }
 
 int rec_half(jsv_plan* P, Adj& A, int l, int par, hipStream_t s)
 {
+    // the half and its snapshot in one launch
     Hier<double>& H = P->h64;
     const Geo& g = H.g[l];
-    dim3 gr = grid_points(g, P->dim, g.mx / 2);
-    if (P->dim == 2)
-        hipLaunchKernelGGL(jsv::k_rb2<double>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
-                           H.cx[l], H.cy[l], H.cc[l], g, par, (const int*)nullptr);
-    else
-        hipLaunchKernelGGL(jsv::k_rb3<double>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
-                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, par, 2);
-    return snap(P, A, l, s);
+    if (A.used[l] >= A.pool[l].size()) return fail("adjoint tape overflow at level %d", l);
+    double* dst = A.pool[l][A.used[l]++];
+    const long n = P->dim == 3 ? g.n : g.sz;
+    hipLaunchKernelGGL(jsv::k_rb_snap, grid_exact(n), dim3(256), 0, s, H.phi[l], dst, H.rhs[l], H.cx[l], H.cy[l],
+                       P->dim == 3 ? H.cz[l] : H.cx[l], H.cc[l], g, par, P->dim);
+    A.tape.push_back({l, dst});
+    return 0;
 }
 
 int rec_smooth(jsv_plan* P, Adj& A, int l, hipStream_t s)

@@ -140,6 +140,31 @@ k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
     rb3_point(phi, rhs, cx, cy, cz, cc, (long)z * g.sz + (long)y * g.sy + x, g.sy, g.sz);
 }
 
+// One RBGS half over the whole level that also snapshots phi after the half
+// (for the adjoint's tape): points of the half are updated and stored to
+// both, the others copied.  Reads never touch points the half writes, so
+// the in-place update has no race.  FP64 only; 2D (dim 2, one problem) or 3D.
+__global__ void __launch_bounds__(256)
+k_rb_snap(double* __restrict__ phi, double* __restrict__ snap, const double* __restrict__ rhs,
+          const double* __restrict__ cx, const double* __restrict__ cy, const double* __restrict__ cz,
+          const double* __restrict__ cc, Geo g, int par, int dim)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const long n = dim == 3 ? g.n : g.sz;
+    if (i >= n) return;
+    const int z = dim == 3 ? (int)(i / g.sz) : 0;
+    const long r = i - (long)z * g.sz;
+    const int y = (int)(r / g.sy), x = (int)(r - (long)y * g.sy);
+    const bool unk = x >= 2 && x <= g.mx && y >= 2 && y <= g.my && (dim == 2 || (z >= 2 && z <= g.mz));
+    // the half's points are x = 2 + ((par + y [+ z]) & 1) + 2k, i.e. x + y [+ z] = par mod 2
+    const int colour = dim == 3 ? ((x + y + z) & 1) : ((x + y) & 1);
+    if (unk && colour == par) {
+        if (dim == 3) rb3_point(phi, rhs, cx, cy, cz, cc, i, g.sy, g.sz);
+        else rb2_point(phi, rhs, cx, cy, cc, i, g.sy);
+    }
+    snap[i] = phi[i];
+}
+
 // Prolongation corrections (solver_2d.py:649-668, solver_3d.py:654-677), shared by
 // the prolongation kernels and the pair sweep that applies the correction on the fly.
 // Correction at fine point (x, y[, z]) from coarse array pc (problem offset
