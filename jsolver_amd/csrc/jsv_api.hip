@@ -634,7 +634,8 @@ long op_points(const jsv_plan* P, const std::vector<Geo>& G, const std::vector<i
     const long nz = P->dim == 3 ? g.mz - e : 1;
     const long full = staged ? ln[op.level] : (P->dim == 3 ? g.n : g.sz);
     switch (op.kind) {
-    case jsv::CO_RB: return (long)(g.mx / 2) * (g.my - e) * nz;
+    case jsv::CO_RB:
+    case jsv::CO_RB_PAIR: return (long)(g.mx / 2) * (g.my - e) * nz;
     case jsv::CO_RB_DIAG: return (long)(g.mx / 2) * (g.my / 2);
     case jsv::CO_ZERO: return full;
     case jsv::CO_PROLONG: return (long)(g.mx - e) * (g.my - e) * nz;
@@ -802,12 +803,27 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         }
         // runs of small ops on staged levels go to one wave
         auto staged = [&](int l) { return l >= lds_from && l < lds_to; };
+        auto solo = [&](const jsv::CoarseOp& op) {
+            const bool two = op.kind == jsv::CO_RESTRICT || op.kind == jsv::CO_PROLONG;
+            return P->coarse_solo > 0 && staged(op.level) && (!two || staged(op.level + 1)) &&
+                   op_points(P, H.g, ln, true, op) <= P->coarse_solo;
+        };
+        {
+            // in a wave's run, the two halves of one sweep share one op
+            std::vector<jsv::CoarseOp> merged;
+            for (size_t i = 0; i < ops.size(); ++i) {
+                const jsv::CoarseOp& a = ops[i];
+                if (i + 1 < ops.size() && a.kind == jsv::CO_RB && ops[i + 1].kind == jsv::CO_RB &&
+                    ops[i + 1].level == a.level && ops[i + 1].a == 1 - a.a && solo(a) && solo(ops[i + 1])) {
+                    merged.push_back({jsv::CO_RB_PAIR, a.level, a.a, 0, 0});
+                    ++i;
+                } else {
+                    merged.push_back(a);
+                }
+            }
+            ops.swap(merged);
+        }
         for (size_t i = 0; i < ops.size();) {
-            auto solo = [&](const jsv::CoarseOp& op) {
-                const bool two = op.kind == jsv::CO_RESTRICT || op.kind == jsv::CO_PROLONG;
-                return P->coarse_solo > 0 && staged(op.level) && (!two || staged(op.level + 1)) &&
-                       op_points(P, H.g, ln, true, op) <= P->coarse_solo;
-            };
             size_t k = i;
             while (k < ops.size() && solo(ops[k])) ++k;
             if (k > i) {

@@ -1054,6 +1054,7 @@ enum CoarseOpKind : int {
     CO_RESTRICT = 2,  // defect of `level` restricted into rhs of level + 1; b = 1: also phi of level + 1 := 0
     CO_ZERO = 3,      // phi of `level` := 0 (whole array incl. rim)
     CO_PROLONG = 4,   // phi of `level` += P phi of level + 1; a = full
+    CO_RB_PAIR = 5,   // both RBGS halves, parity a first (one-wave runs only)
 };
 
 struct CoarseOp {
@@ -1154,27 +1155,37 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
     // extents of the loops: exact on global levels, padded to powers of two on LDS ones
     auto ext = [&](int m) { return FL ? m : m - 1; };
     auto lg = [](int v) { return v > 0 ? __builtin_ctz((unsigned)v) : 0; };
-    if (op.kind == CO_RB) {
+    if (op.kind == CO_RB || op.kind == CO_RB_PAIR) {
         const int xh = g.mx / 2, ny = ext(g.my), lxh = lg(xh), lny = lg(ny);
-        if constexpr (dim == 3) {
-            const int cnt = xh * ny * ext(g.mz);
-            for (int i = t0; i < cnt; i += nt) {
-                const Ix2 zr = split2<FL>(i, xh * ny, lxh + lny);
-                const Ix2 yx = split2<FL>(zr.lo, xh, lxh);
-                const int z = 2 + zr.hi, y = 2 + yx.hi;
-                const int x = 2 + ((op.a + y + z) & 1) + 2 * yx.lo;
-                if (x > g.mx || y > g.my || z > g.mz) continue;
-                rb3_point(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, F.at(z, y, x), F.sy, F.sz);
+        auto half = [&](int par) {
+            if constexpr (dim == 3) {
+                const int cnt = xh * ny * ext(g.mz);
+                for (int i = t0; i < cnt; i += nt) {
+                    const Ix2 zr = split2<FL>(i, xh * ny, lxh + lny);
+                    const Ix2 yx = split2<FL>(zr.lo, xh, lxh);
+                    const int z = 2 + zr.hi, y = 2 + yx.hi;
+                    const int x = 2 + ((par + y + z) & 1) + 2 * yx.lo;
+                    if (x > g.mx || y > g.my || z > g.mz) continue;
+                    rb3_point(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, F.at(z, y, x), F.sy, F.sz);
+                }
+            } else {
+                const int cnt = xh * ny;
+                for (int i = t0; i < cnt; i += nt) {
+                    const Ix2 yx = split2<FL>(i, xh, lxh);
+                    const int y = 2 + yx.hi;
+                    const int x = 2 + ((par + y) & 1) + 2 * yx.lo;
+                    if (x > g.mx || y > g.my) continue;
+                    rb2_point(F.phi, F.rhs, F.cx, F.cy, F.cc, F.at(y, x), F.sy);
+                }
             }
-        } else {
-            const int cnt = xh * ny;
-            for (int i = t0; i < cnt; i += nt) {
-                const Ix2 yx = split2<FL>(i, xh, lxh);
-                const int y = 2 + yx.hi;
-                const int x = 2 + ((op.a + y) & 1) + 2 * yx.lo;
-                if (x > g.mx || y > g.my) continue;
-                rb2_point(F.phi, F.rhs, F.cx, F.cy, F.cc, F.at(y, x), F.sy);
-            }
+        };
+        half(op.a);
+        if (op.kind == CO_RB_PAIR) {
+            // one wave (runs only): the second half reads what other lanes wrote
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            half(1 - op.a);
         }
     } else if (op.kind == CO_RB_DIAG) {
         const int xh = g.mx / 2, yh = g.my / 2, lxh = lg(xh);
