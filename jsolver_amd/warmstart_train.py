@@ -170,11 +170,11 @@ def smooth_guess(phi0, rhs, lam, dxx, dyy, sweeps):
 
 
 def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_cycles=50000,
-             guess_sweeps=4, judge="rhs"):
+             guess_sweeps=4, criterion="rhs"):
     """Cycles and wall time to rtol from a zero start and from the network's
     guess after `guess_sweeps` RBGS sweeps, with zero-start fallback per
-    problem.  judge="rhs": fall back when ||rhs - L phi0|| > ||rhs|| (the
-    residual of the zero start).  judge="cycle": run one cycle from the guess
+    problem.  criterion="rhs": fall back when ||rhs - L phi0|| > ||rhs|| (the
+    residual of the zero start).  criterion="cycle": run one cycle from the guess
     and one from zero and fall back when the guess's residual after it is the
     larger; that cycle pair is counted in the warm-start cycles.  Problems on
     which either solve does not converge are excluded from the cycle
@@ -193,7 +193,7 @@ def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_
     if guess_sweeps > 0:
         phi0 = smooth_guess(phi0, rhs, lam, dxx, dyy, guess_sweeps)
     extra = 0
-    if judge == "rhs":
+    if criterion == "rhs":
         # ||rhs - L phi0|| vs ||rhs|| per problem
         _, _, rel0 = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, fixed=True, max_iters_outer=0,
                                       stop="rel_residual", rtol=rtol, precision="mixed")
@@ -216,13 +216,13 @@ def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_
     t_warm = time.perf_counter() - t0
     good = np.isfinite(r0) & (r0 <= rtol) & np.isfinite(r1) & (r1 <= rtol)
     c0, c1 = np.asarray(c0)[good], np.asarray(c1)[good]
-    res = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol, "guess_sweeps": guess_sweeps, "judge": judge,
+    res = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol, "guess_sweeps": guess_sweeps, "criterion": criterion,
            "unconverged_excluded": int((~good).sum()),
            "cycles_zero_mean": float(np.mean(c0)), "cycles_warm_mean": float(np.mean(c1)),
            "cycle_reduction": float(1.0 - np.sum(c1) / np.sum(c0)),
            "per_problem_reduction_median": float(np.median(1.0 - c1 / c0)),
            "time_zero_s": t_zero, "time_warm_s": t_warm,
-           "guess_judge_ratio_median": float(np.median(rel0)),
+           "guess_criterion_ratio_median": float(np.median(rel0)),
            "fallbacks": int((~use).sum().item()),
            "max_final_rel_residual": float(max(np.max(r0[good]), np.max(r1[good])))}
     log(f"[eval] {json.dumps(res)}")
@@ -255,11 +255,11 @@ def main(argv=None):
         net = UNet(seed=a.seed, device=dev)
         net.load_state_dict(torch.load(a.eval_only, weights_only=True))
         out = {}
-        for judge in ("rhs", "cycle"):
+        for criterion in ("rhs", "cycle"):
             for sw in sorted({0, a.guess_sweeps}):
-                out[f"{judge}_sweeps{sw}"] = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev, gamma=a.gamma,
+                out[f"{criterion}_sweeps{sw}"] = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev, gamma=a.gamma,
                                                       log=log, max_cycles=a.max_cycles, guess_sweeps=sw,
-                                                      judge=judge)
+                                                      criterion=criterion)
         print(json.dumps(out), flush=True)
         return
     t0 = time.perf_counter()
