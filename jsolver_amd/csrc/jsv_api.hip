@@ -1713,7 +1713,7 @@ static int solve_fp64(jsv_plan* P, const double* phi_in, const double* rhs, doub
         // the predicted cycles-to-tolerance (check_every = 0, rel_residual
         // only).  Each problem keeps its own check schedule: every active
         // problem runs every cycle, and at a check only the problems due are
-        // judged, as in their single solves.
+        // tested against the tolerance, as in their single solves.
         const bool adaptive = relres && c->check_every == 0;
         const int64_t every = relres ? (c->check_every > 0 ? c->check_every : 1) : 1;
         const int64_t first_block = adaptive ? 8 : every;
