@@ -232,6 +232,22 @@ int jsv_adjoint(jsv_plan *plan, const double *phi0, const double *rhs, const dou
 /* device bytes held by the plan's reverse-pass buffers (tape included) */
 int jsv_adjoint_device_bytes(const jsv_plan *plan, int64_t *bytes);
 
+/* ---- tangent (forward-mode) pass of solves (FP64) ----
+ * Replaces the reference's jax.jvp over a solve (examples/benchmark.py:1146-1182;
+ * tests/test_solver_2d.py:173-176, check_grads(modes=["fwd"])).  Runs
+ * cfg->max_iters cycles (with cfg's cycle shape; cfg->fixed and the stop rule
+ * are ignored: a dynamic solve's tangent is that of its cycle count) on the
+ * primal (phi0, rhs) with plan's operator, and alongside them the tangent
+ * cycles driven by (phi0_dot, rhs_dot) and by the tangent operator held by
+ * tangent_plan: a second plan of the same shape whose operator was set from
+ * the tangents (D_dot, lambda_dot[, D_xy_dot]) with jsv_plan_set_operator (the
+ * coefficient setup is linear in D and lambda).  phi0_dot / rhs_dot may be
+ * NULL (zero); phi_out may be NULL.  Both plans' level buffers are
+ * overwritten.  Unbatched plans; D_xy supported. */
+int jsv_tangent(jsv_plan *plan, jsv_plan *tangent_plan, const double *phi0, const double *rhs,
+                const double *phi0_dot, const double *rhs_dot, const jsv_config *cfg,
+                double *phi_out, double *phi_dot_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,7 @@ SIGNATURES = {
     "jsv_forward_checkpoints": (_I, [_P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P]),
     "jsv_adjoint": (_I, [_P, _P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P, _P, _P, _P, _P, _P, _P]),
     "jsv_adjoint_device_bytes": (_I, [_P, ctypes.POINTER(_I64)]),
+    "jsv_tangent": (_I, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P, _P]),
 }
 
 _lock = threading.Lock()

@@ -111,8 +111,10 @@ _PLANS = OrderedDict()
 _PLAN_CACHE_SIZE = 4
 
 
-def get_plan(dim, ms, idels, precision, batch=1):
-    key = (dim, tuple(int(m) for m in ms), tuple(float(i) for i in idels), precision, int(batch))
+def get_plan(dim, ms, idels, precision, batch=1, role="primal"):
+    """Cached plan; role "tangent" is a second plan of the same shape that
+    holds the tangent operator of a forward-mode solve."""
+    key = (dim, tuple(int(m) for m in ms), tuple(float(i) for i in idels), precision, int(batch), role)
     p = _PLANS.get(key)
     if p is None:
         p = Plan(dim, ms, idels, precision, batch)
@@ -198,8 +200,19 @@ def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12
         raise ValueError(f"Shape mismatch: phi={shape_of(phi)} does not match the grids "
                          f"(expected {full_shape}).")
     dev = device()
-    plan = get_plan(dim, ms, [g.idel for g in grids], precision)
     args = [phi, rhs, lam, *D] + ([D_xy] if D_xy is not None else [])
+    from . import tangent
+    if any(tangent.has_tangent(v) for v in args):
+        if return_history:
+            raise ValueError("return_history is not available for differentiated solves")
+        out, count, info = tangent.solve_forward_ad(
+            dim, grids, phi, rhs, lam, D, D_xy,
+            dict(fixed=fixed, epsilon=epsilon, gamma=gamma, fcycle=fcycle, skip_pre=skip_pre, nu1=nu1,
+                 nu2=nu2, max_iters_outer=max_iters_outer, precision=precision, stop=stop, rtol=rtol,
+                 inner_cycles=inner_cycles, check_every=check_every),
+            (dev, tuple(int(m) for m in ms), tuple(float(g.idel) for g in grids)))
+        return (out, count, info) if return_info else (out, count)
+    plan = get_plan(dim, ms, [g.idel for g in grids], precision)
     if torch.is_grad_enabled() and any(torch.is_tensor(v) and v.requires_grad for v in args):
         return _solve_differentiable(dim, plan, dev, full_shape, phi, rhs, lam, D, D_xy, fixed,
                                      precision, dict(fixed=True, epsilon=epsilon, gamma=gamma,

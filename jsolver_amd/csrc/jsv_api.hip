@@ -2309,3 +2309,124 @@ int jsv_adjoint_device_bytes(const jsv_plan* P, int64_t* bytes)
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Tangent (forward-mode) pass (FP64): the primal cycle and its tangent in
+// lockstep, one launch per operation on both
+// ===========================================================================
+#include "jsv_tangent.hpp"
+
+namespace {
+
+jsv::TanCoef tan_coef(const jsv_plan* T, int l)
+{
+    const Hier<double>& G = T->h64;
+    return {G.cx[l], G.cy[l], T->dim == 3 ? G.cz[l] : nullptr, G.cc[l],
+            T->offdiag ? G.drr[l] : nullptr, T->offdiag ? G.dlr[l] : nullptr};
+}
+
+int tan_smooth(jsv_plan* P, jsv_plan* T, int l, hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    Hier<double>& G = T->h64;
+    const Geo& g = H.g[l];
+    const jsv::TanCoef tc = tan_coef(T, l);
+    if (P->dim == 2 && P->offdiag) {
+        // the reference's sub-colour order (0,0),(1,1),(0,1),(1,0)
+        static const int order[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+        const dim3 gr((unsigned)((g.mx / 2 + jsv::BX - 1) / jsv::BX),
+                      (unsigned)((g.my / 2 + jsv::BY - 1) / jsv::BY), 1);
+        for (int k = 0; k < 4; ++k)
+            hipLaunchKernelGGL(jsv::k_rb2diag_tan, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], G.phi[l],
+                               H.rhs[l], G.rhs[l], H.cx[l], H.cy[l], H.cc[l], H.drr[l], H.dlr[l], tc, g,
+                               order[k][0], order[k][1]);
+        return 0;
+    }
+    const dim3 gr = grid_points(g, P->dim, g.mx / 2);
+    const int f = P->dim == 3 ? 1 : 0;   // first half: x+y+z odd (3D), x+y even (2D)
+    for (int par : {f, 1 - f})
+        hipLaunchKernelGGL(jsv::k_rb_tan, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], G.phi[l], H.rhs[l],
+                           G.rhs[l], H.cx[l], H.cy[l], P->dim == 3 ? H.cz[l] : nullptr, H.cc[l], tc, g,
+                           par, P->dim);
+    return 0;
+}
+
+// multigrid_cycle (solver_2d.py:582-614 / solver_3d.py:588-620) with its tangent
+int tan_cycle(jsv_plan* P, jsv_plan* T, const jsv_config& c, int l, bool fcyc, bool skip_pre,
+              hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    Hier<double>& G = T->h64;
+    if (l == P->levels - 2) {
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) RETCHK(tan_smooth(P, T, l, s));
+        return 0;
+    }
+    if (!skip_pre)
+        for (int i = 0; i < c.nu1; ++i) RETCHK(tan_smooth(P, T, l, s));
+    const Geo& gf = H.g[l];
+    const Geo& gc = H.g[l + 1];
+    {
+        // the tangent restriction reads the primal phi of level l, which the
+        // primal restriction leaves alone
+        const dim3 gr = grid_points(gc, P->dim, gc.mx - 1);
+        const bool od = P->dim == 2 && P->offdiag;
+        hipLaunchKernelGGL(jsv::k_restrict_tan, gr, dim3(jsv::BX, jsv::BY), 0, s, G.rhs[l + 1], H.phi[l],
+                           G.phi[l], G.rhs[l], H.cx[l], H.cy[l], P->dim == 3 ? H.cz[l] : nullptr, H.cc[l],
+                           od ? H.drr[l] : nullptr, od ? H.dlr[l] : nullptr, tan_coef(T, l), gf, gc, P->dim,
+                           G.phi[l + 1]);
+    }
+    RETCHK(restrict_(P, H, l, s, H.phi[l + 1]));
+    if (fcyc)
+        for (int i = 0; i < c.gamma; ++i) RETCHK(tan_cycle(P, T, c, l + 1, true, false, s));
+    for (int i = 0; i < c.gamma; ++i) RETCHK(tan_cycle(P, T, c, l + 1, false, false, s));
+    RETCHK(prolong(P, H, l, c.nu2, s));
+    RETCHK(prolong(T, G, l, c.nu2, s));
+    for (int i = 0; i < c.nu2; ++i) RETCHK(tan_smooth(P, T, l, s));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jsv_tangent(jsv_plan* P, jsv_plan* T, const double* phi0, const double* rhs, const double* phi0_dot,
+                const double* rhs_dot, const jsv_config* c, double* phi_out, double* phi_dot_out,
+                void* stream)
+{
+    if (P == nullptr || T == nullptr || !P->op_set || !T->op_set)
+        return fail("plan missing or operator not set");
+    if (phi0 == nullptr || rhs == nullptr || phi_dot_out == nullptr) return fail("NULL argument");
+    if (P == T) return fail("the tangent needs its own plan");
+    if (P->dim != T->dim || P->levels != T->levels || P->m[0] != T->m[0] || P->m[1] != T->m[1] ||
+        P->m[2] != T->m[2])
+        return fail("primal and tangent plans differ in shape");
+    if (P->batch != 1 || T->batch != 1) return fail("the tangent pass needs unbatched plans");
+    if (P->offdiag != T->offdiag) return fail("primal and tangent plans differ in D_xy");
+    RETCHK(check_cfg(c));
+    hipStream_t s = (hipStream_t)stream;
+    Hier<double>& H = P->h64;
+    Hier<double>& G = T->h64;
+    const long n = H.g[0].n;
+    HIPCHK(hipMemcpyAsync(H.rhs[0], rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(H.phi[0], phi0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (rhs_dot) HIPCHK(hipMemcpyAsync(G.rhs[0], rhs_dot, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    else RETCHK(zero(G.rhs[0], n, s));
+    if (phi0_dot) HIPCHK(hipMemcpyAsync(G.phi[0], phi0_dot, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    else RETCHK(zero(G.phi[0], n, s));
+    RETCHK(zero_rim_level0(P, H.phi[0], s));
+    RETCHK(zero_rim_level0(P, G.phi[0], s));
+    if (c->skip_pre)
+        for (int i = 0; i < c->nu1; ++i) RETCHK(tan_smooth(P, T, 0, s));
+    for (int64_t k = 0; k < c->max_iters; ++k) {
+        RETCHK(zero_rim_level0(P, H.phi[0], s));
+        RETCHK(zero_rim_level0(P, G.phi[0], s));
+        RETCHK(tan_cycle(P, T, *c, 0, c->fcycle != 0, c->skip_pre != 0, s));
+        LAUNCHCHK();
+    }
+    if (phi_out) HIPCHK(hipMemcpyAsync(phi_out, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(phi_dot_out, G.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    LAUNCHCHK();
+    return 0;
+}
+
+}  // extern "C"

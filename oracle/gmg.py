@@ -306,7 +306,8 @@ def build_operator(dim, ms, idel, lam, D, D_xy=None) -> Operator:
                     "rr": (w_xp + w_yp) * c,
                 }
     else:
-        Dv = [np.float64(D[dim - 1 - a]) for a in range(dim)]
+        # np.float64 scalars (complex128 under a complex-step perturbation)
+        Dv = [np.asarray(D[dim - 1 - a], dtype=_ft(D[dim - 1 - a]))[()] for a in range(dim)]
         for l in range(levels):
             a_axis = [Dv[a] * sq_ax[a][l] for a in range(dim)]
             # add = 2 * (A_x + A_y [+ A_z]), fastest axis first

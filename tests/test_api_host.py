@@ -140,6 +140,32 @@ def test_compute_fails_loudly_without_gpu():
         solve_2d(g, g, a, a, 0.0, 1.0, 1.0)
 
 
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU failure")
+def test_forward_mode_fails_loudly_without_gpu():
+    """A dual tensor routes the solve to the tangent path, which also needs the GPU."""
+    import torch.autograd.forward_ad as fwAD
+    g = grid_1D(16)
+    a = torch.zeros((19, 19), dtype=torch.float64)
+    with fwAD.dual_level():
+        r = fwAD.make_dual(a, torch.ones_like(a))
+        with pytest.raises(_lib.JsvError, match="GPU"):
+            solve_2d(g, g, a, r, 0.0, 1.0, 1.0)
+
+
+def test_tangent_detection():
+    """Forward-mode tangents are recognised on dual tensors and inside
+    torch.func.jvp, and only there."""
+    import torch.autograd.forward_ad as fwAD
+    from jsolver_amd.tangent import has_tangent
+    assert not has_tangent(np.zeros(3)) and not has_tangent(1.0) and not has_tangent(torch.zeros(3))
+    with fwAD.dual_level():
+        assert has_tangent(fwAD.make_dual(torch.zeros(3), torch.ones(3)))
+        assert not has_tangent(torch.zeros(3))
+    seen = []
+    torch.func.jvp(lambda x: (seen.append(has_tangent(x)), 2 * x)[1], (torch.zeros(2),), (torch.ones(2),))
+    assert seen == [True]
+
+
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=300)
