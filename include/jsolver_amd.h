@@ -97,7 +97,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         second pass (default 2^20; 2^13 for 3D FP32 levels;
  *                         the tuned value applies to both precisions)
  *   JSV_TUNE_FUSED_MIN_M  3D levels with mz >= this run each RBGS sweep as one
- *                         fused out-of-place z-marching kernel (default 128;
+ *                         fused out-of-place z-marching kernel (default 256;
  *                         0 = never)
  *   JSV_TUNE_ZCHUNK       planes per workgroup of that kernel (0 = by level:
  *                         one wave of workgroups for the pair kernel, 16 in
@@ -131,7 +131,14 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         this many points run in one wave without workgroup
  *                         barriers (default 64, at most 64; 0 = never)
  *   JSV_TUNE_COARSE_THREADS workgroup size of that launch: 256, 512 or 1024
- *                         (default 1024) */
+ *                         (default 1024)
+ *   JSV_TUNE_FUSE_HALVES  levels smoothed with one launch per RBGS half (2D
+ *                         without D_xy; 3D below JSV_TUNE_FUSED_MIN_M): 1
+ *                         (default) the restriction into such a level also
+ *                         runs the first half of its first sweep (its phi is
+ *                         zero then), and the first half of the
+ *                         post-smoothing applies the prolongation on the fly;
+ *                         0: separate launches */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -146,6 +153,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_SWEEP_PAIRS 11
 #define JSV_TUNE_COARSE_SOLO 12
 #define JSV_TUNE_COARSE_THREADS 13
+#define JSV_TUNE_FUSE_HALVES 14
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

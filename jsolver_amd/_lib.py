@@ -28,6 +28,7 @@ JSV_TUNE_COARSE_LDS = 10
 JSV_TUNE_SWEEP_PAIRS = 11
 JSV_TUNE_COARSE_SOLO = 12
 JSV_TUNE_COARSE_THREADS = 13
+JSV_TUNE_FUSE_HALVES = 14
 JSV_BUF_PHI = 0
 JSV_BUF_RHS = 1
 

@@ -113,7 +113,7 @@ struct jsv_plan {
     int* h_actk = nullptr;          // pinned staging of d_actk
     long slab_bytes = 0;            // slab size of the 3D two-launch smoothing schedule (0: whole level)
     long two_pass_min[2] = {1L << 20, 1L << 20};   // FP64 / FP32: level size from which restriction runs in two passes
-    int fused_min_m = 128;          // 3D levels with mz >= this use the fused out-of-place sweep
+    int fused_min_m = 256;          // 3D levels with mz >= this use the fused out-of-place sweep
     int zchunk = 0;                 // planes per workgroup of the fused sweep (0: by level size)
     int sweep_rows = 0;             // rows per workgroup tile of the fused sweep (0: by precision)
     int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024)
@@ -121,6 +121,8 @@ struct jsv_plan {
     int sweep_pairs = 0;            // pair kernel: column pairs per thread (0: 2 in FP32, 1 in FP64)
     int n_cu = 256;                 // compute units of the device
     int fuse_prolong = 2;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
+    int fuse_halves = 1;            // levels smoothed one launch per half: the restriction runs the coarse
+                                    // level's first half, the first post-smoothing half applies the prolongation
     bool use_graphs = true;         // replay whole cycles as captured hipGraphs
     int coarse_m[2] = {0, 0};       // FP64 / FP32: levels with max resolution <= this run as one coarse program (0: off)
     void* lv_dev[2] = {nullptr, nullptr};          // LvPtr<double> / LvPtr<float> per level
@@ -411,10 +413,25 @@ bool prolong_fusable(const jsv_plan* P, const Hier<T>& H, int l)
     return P->sweep_variant == 2 && pair_sweep_fits(g, sizeof(T), &H.g[l + 1]);
 }
 
-// pc: coarse correction to prolongate into level l first (3D fused pair sweep
-// only; see prolong_fusable)
+// Level l smooths with one launch per RBGS half in the plain order (2D
+// without D_xy; 3D below the fused-sweep size without the slab schedule).
 template <typename T>
-int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr)
+bool halves_path(const jsv_plan* P, const Hier<T>& H, int l)
+{
+    if (P->dim == 2) return !P->offdiag;
+    const Geo& g = H.g[l];
+    if (P->fused_min_m > 0 && g.mz >= P->fused_min_m && H.phi2[l] != nullptr) return false;
+    const int planes = g.mz - 1;
+    const long plane_bytes = (long)g.sz * sizeof(T) * 6;
+    return !(P->slab_bytes > 0 && plane_bytes * planes > P->slab_bytes * 2);
+}
+
+// pc: coarse correction to prolongate into level l first (3D fused pair sweep,
+// see prolong_fusable; or, with fuse_halves, the first half of the halves path).
+// red_done: the first half already ran inside the restriction that produced
+// this level (fuse_halves; halves path only), so only the second half runs.
+template <typename T>
+int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr, bool red_done = false)
 {
     const Geo& g = H.g[l];
     if (P->dim == 2) {
@@ -432,8 +449,12 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
             }
         } else {
             dim3 gr = grid_points(g, 2, g.mx / 2);
-            hipLaunchKernelGGL(jsv::k_rb2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
-                               H.rhs[l], H.cx[l], H.cy[l], H.cc[l], g, 0, P->act_ptr());
+            if (pc != nullptr)
+                hipLaunchKernelGGL(jsv::k_rb2_pro<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
+                                   H.cx[l], H.cy[l], H.cc[l], g, 0, P->act_ptr(), pc, H.g[l + 1]);
+            else if (!red_done)
+                hipLaunchKernelGGL(jsv::k_rb2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
+                                   H.rhs[l], H.cx[l], H.cy[l], H.cc[l], g, 0, P->act_ptr());
             hipLaunchKernelGGL(jsv::k_rb2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
                                H.rhs[l], H.cx[l], H.cy[l], H.cc[l], g, 1, P->act_ptr());
         }
@@ -456,6 +477,15 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
             if (slab < 2) slab = 2;
         }
         dim3 gr = grid_points(g, 3, g.mx / 2);
+        if (slab == planes && (pc != nullptr || red_done)) {
+            // halves path with a fused first half
+            if (pc != nullptr)
+                hipLaunchKernelGGL(jsv::k_rb3_pro<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
+                                   H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, 1, pc, H.g[l + 1]);
+            hipLaunchKernelGGL(jsv::k_rb3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l], H.cx[l],
+                               H.cy[l], H.cz[l], H.cc[l], g, 0, 2);
+            return 0;
+        }
         auto launch = [&](int par, int za, int zb) {
             dim3 gs(gr.x, gr.y, (unsigned)(zb - za));
             hipLaunchKernelGGL(jsv::k_rb3<T>, gs, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
@@ -475,12 +505,16 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
     return 0;
 }
 
-// zphi: the coarse phi to zero in the same kernels (nullptr: left alone)
+// zphi: the coarse phi to zero in the same kernels (nullptr: left alone);
+// red_first: also run the coarse level's first RBGS half (zphi must be set)
 template <typename T>
-int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nullptr)
+int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nullptr, bool red_first = false)
 {
     const Geo& gf = H.g[l];
     const Geo& gc = H.g[l + 1];
+    jsv::RedZ<T> rz{nullptr, nullptr, nullptr, nullptr};
+    if (red_first && zphi != nullptr)
+        rz = {H.cx[l + 1], H.cy[l + 1], P->dim == 3 ? H.cz[l + 1] : nullptr, H.cc[l + 1]};
     dim3 gr = grid_points(gc, P->dim, gc.mx - 1);
     if (H.res[l] != nullptr) {
         // two passes: residual at the first-half points, then the weighting
@@ -491,12 +525,12 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
                                P->offdiag ? H.drr[l] : nullptr, P->offdiag ? H.dlr[l] : nullptr, gf,
                                P->act_ptr());
             hipLaunchKernelGGL(jsv::k_restrict_gather2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
-                               H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr(), zphi);
+                               H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr(), zphi, rz);
         } else {
             hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
                                H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf);
             hipLaunchKernelGGL(jsv::k_restrict_gather3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
-                               H.rhs[l + 1], H.res[l], gf, gc, zphi);
+                               H.rhs[l + 1], H.res[l], gf, gc, zphi, rz);
         }
         return 0;
     }
@@ -504,10 +538,10 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
         hipLaunchKernelGGL(jsv::k_restrict2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
                            H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cc[l],
                            P->offdiag ? H.drr[l] : nullptr, P->offdiag ? H.dlr[l] : nullptr, gf, gc,
-                           P->act_ptr(), zphi);
+                           P->act_ptr(), zphi, rz);
     else
         hipLaunchKernelGGL(jsv::k_restrict3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
-                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, gc, zphi);
+                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, gc, zphi, rz);
     return 0;
 }
 
@@ -845,21 +879,27 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
 }
 
 // multigrid_cycle (solver_2d.py:582-614 / solver_3d.py:588-620)
+// red_done: the restriction into this level already ran the first half of
+// its first sweep (fuse_halves)
 template <typename T>
 int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcyc, bool skip_pre,
-             hipStream_t s)
+             hipStream_t s, bool red_done = false)
 {
     if (l == P->levels - 2) {
-        for (int i = 0; i < c.nu1 + c.nu2; ++i) RETCHK(smooth(P, H, l, s));
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0));
         return 0;
     }
     if (!skip_pre)
-        for (int i = 0; i < c.nu1; ++i) RETCHK(smooth(P, H, l, s));
+        for (int i = 0; i < c.nu1; ++i) RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0));
     const int start = coarse_start(P, H);
     const jsv_plan::CoarseProg* pr = (start == l + 1) ? find_prog<T>(P, c, fcyc, start) : nullptr;
     if (pr != nullptr && pr->ops == nullptr) pr = nullptr;
+    // the first visit of level l + 1 starts from phi = 0 with a sweep: its first
+    // half can run inside the restriction
+    const int first_sweeps = (l + 1 == P->levels - 2) ? c.nu1 + c.nu2 : c.nu1;
+    const bool red_first = pr == nullptr && P->fuse_halves && first_sweeps >= 1 && halves_path(P, H, l + 1);
     // phi of level l + 1 := 0 happens inside the restriction (or the program)
-    RETCHK(restrict_(P, H, l, s, pr != nullptr ? (T*)nullptr : H.phi[l + 1]));
+    RETCHK(restrict_(P, H, l, s, pr != nullptr ? (T*)nullptr : H.phi[l + 1], red_first));
     if (pr != nullptr) {
         // zero + the gamma (F: 2 gamma) sub-cycles of level l + 1 in one launch
         const int pi = sizeof(T) == 4 ? 1 : 0;
@@ -886,11 +926,18 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
             : nt == 512 ? launch(jsv::k_coarse_prog<T, 512, 3>, nt)
                         : launch(jsv::k_coarse_prog<T, 1024, 3>, nt);
     } else {
+        bool first = red_first;
         if (fcyc)
-            for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, true, false, s));
-        for (int i = 0; i < c.gamma; ++i) RETCHK(mg_cycle(P, H, c, l + 1, false, false, s));
+            for (int i = 0; i < c.gamma; ++i) {
+                RETCHK(mg_cycle(P, H, c, l + 1, true, false, s, first));
+                first = false;
+            }
+        for (int i = 0; i < c.gamma; ++i) {
+            RETCHK(mg_cycle(P, H, c, l + 1, false, false, s, first));
+            first = false;
+        }
     }
-    if (c.nu2 >= 1 && prolong_fusable(P, H, l)) {
+    if (c.nu2 >= 1 && (prolong_fusable(P, H, l) || (P->fuse_halves && halves_path(P, H, l)))) {
         // the first post-smoothing sweep applies the prolongation itself
         RETCHK(smooth(P, H, l, s, (const T*)H.phi[l + 1]));
         for (int i = 1; i < c.nu2; ++i) RETCHK(smooth(P, H, l, s));
@@ -1247,6 +1294,10 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     }
     if (key == JSV_TUNE_ZCHUNK) {
         P->zchunk = (int)value;
+        return 0;
+    }
+    if (key == JSV_TUNE_FUSE_HALVES) {
+        P->fuse_halves = value != 0 ? 1 : 0;
         return 0;
     }
     if (key == JSV_TUNE_FUSE_PROLONG) {

@@ -250,6 +250,72 @@ __device__ __forceinline__ bool prolong3_corr(const T* __restrict__ pc, long csy
                                x, y, z, full, corr);
 }
 
+// First RBGS half of the post-smoothing with the prolongation phi += P e
+// (solver_2d.py:612, 649-668; solver_3d.py:618, 654-677) applied on the fly:
+// this half reads only the other half's values, each taken as stored value
+// plus its correction (computed as the prolongation kernels do, unknowns
+// only), and with nu2 > 0 nothing reads the other half's corrected values
+// later (the second half overwrites them without reading them) nor this
+// half's own old values.  Same bits as k_prolong3 / k_prolong2 then the half.
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_rb3_pro(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
+          const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc, Geo g, int par,
+          const T* __restrict__ pc, Geo gc)
+{
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int z = 2 + blockIdx.z;
+    if (y > g.my) return;
+    const int x = 2 + ((par + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (x > g.mx) return;
+    const long sy = g.sy, sz = g.sz;
+    const long p = (long)z * sz + (long)y * sy + x;
+    auto nb = [&](int xx, int yy, int zz, long q) -> T {
+        T v = phi[q];
+        T corr;
+        if (xx >= 2 && xx <= g.mx && yy >= 2 && yy <= g.my && zz >= 2 && zz <= g.mz &&
+            prolong3_corr(pc, gc.sy, gc.sz, xx, yy, zz, 0, corr))
+            v = v + corr;
+        return v;
+    };
+    T acc = cz[p] * nb(x, y, z + 1, p + sz) + cz[p - sz] * nb(x, y, z - 1, p - sz);
+    acc = acc + cy[p] * nb(x, y + 1, z, p + sy);
+    acc = acc + cy[p - sy] * nb(x, y - 1, z, p - sy);
+    acc = acc + cx[p] * nb(x + 1, y, z, p + 1);
+    acc = acc + cx[p - 1] * nb(x - 1, y, z, p - 1);
+    acc = acc - rhs[p];
+    phi[p] = acc / cc[p];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_rb2_pro(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
+          const T* __restrict__ cy, const T* __restrict__ cc, Geo g, int par, const int* __restrict__ act,
+          const T* __restrict__ pc, Geo gc)
+{
+    const int b = blockIdx.z;
+    if (act != nullptr && act[b] == 0) return;
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    if (y > g.my) return;
+    const int x = 2 + ((par + y) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (x > g.mx) return;
+    const long sy = g.sy;
+    const long p = (long)b * g.sz + (long)y * sy + x;
+    const T* pcb = pc + (long)b * gc.sz;
+    auto nb = [&](int xx, int yy, long q) -> T {
+        T v = phi[q];
+        T corr;
+        if (xx >= 2 && xx <= g.mx && yy >= 2 && yy <= g.my && prolong2_corr(pcb, gc.sy, xx, yy, 0, corr))
+            v = v + corr;
+        return v;
+    };
+    T acc = cy[p] * nb(x, y + 1, p + sy) + cy[p - sy] * nb(x, y - 1, p - sy);
+    acc = acc + cx[p] * nb(x + 1, y, p + 1);
+    acc = acc + cx[p - 1] * nb(x - 1, y, p - 1);
+    acc = acc - rhs[p];
+    phi[p] = acc / cc[p];
+}
+
 // ---------------------------------------------------------------------------
 // Fused RBGS sweep (red half then black half), out of place, marching in z.
 //
@@ -833,6 +899,42 @@ __device__ __forceinline__ T restrict3_at(const T* __restrict__ phi, const T* __
     return (T)0.03125 * e + (T)0.125 * rcen;
 }
 
+// The first RBGS half of the level a restriction produces, run inside the
+// restriction kernel.  That level's phi is zero when its first visit starts,
+// so the first half's new value at a point needs only the point's own rhs
+// (just computed by the same thread) and coefficients: the expression of
+// rb2_point / rb3_point with every neighbour value an exact zero (products
+// kept, so NaN and Inf coefficients propagate as in the separate launch).
+// cc == nullptr: disabled.
+template <typename T>
+struct RedZ {
+    const T *cx, *cy, *cz, *cc;
+};
+
+template <typename T>
+__device__ __forceinline__ T rb2_from_zero(T rc, const RedZ<T>& z, long p, long sy)
+{
+    const T o = (T)0;
+    T acc = z.cy[p] * o + z.cy[p - sy] * o;
+    acc = acc + z.cx[p] * o;
+    acc = acc + z.cx[p - 1] * o;
+    acc = acc - rc;
+    return acc / z.cc[p];
+}
+
+template <typename T>
+__device__ __forceinline__ T rb3_from_zero(T rc, const RedZ<T>& z, long p, long sy, long sz)
+{
+    const T o = (T)0;
+    T acc = z.cz[p] * o + z.cz[p - sz] * o;
+    acc = acc + z.cy[p] * o;
+    acc = acc + z.cy[p - sy] * o;
+    acc = acc + z.cx[p] * o;
+    acc = acc + z.cx[p - 1] * o;
+    acc = acc - rc;
+    return acc / z.cc[p];
+}
+
 // Zero the coarse phi that the cycle visits next, in the restriction kernel
 // instead of a separate fill: the thread of coarse unknown (I, J, K) clears it
 // and the rim points whose nearest unknown it is, so the whole array ends zero.
@@ -854,7 +956,7 @@ __global__ void __launch_bounds__(BX * BY)
 k_restrict2(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cc,
             const T* __restrict__ drr, const T* __restrict__ dlr, Geo gf, Geo gc,
-            const int* __restrict__ act, T* __restrict__ zphi)
+            const int* __restrict__ act, T* __restrict__ zphi, RedZ<T> rz)
 {
     const int b = blockIdx.z;
     if (act != nullptr && act[b] == 0) return;
@@ -862,23 +964,29 @@ k_restrict2(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__
     const int K = 2 + blockIdx.x * BX + threadIdx.x;
     if (J > gc.my || K > gc.mx) return;
     const long c = (long)b * gf.sz + (long)(2 * J - 1) * gf.sy + (2 * K - 1);
-    rc[(long)b * gc.sz + (long)J * gc.sy + K] = restrict2_at(phi, rhs, cx, cy, cc, drr, dlr, c, gf.sy);
+    const long q = (long)b * gc.sz + (long)J * gc.sy + K;
+    const T v = restrict2_at(phi, rhs, cx, cy, cc, drr, dlr, c, gf.sy);
+    rc[q] = v;
     if (zphi != nullptr) zero_with_rim(zphi, gc, (long)b * gc.sz, 2, 0, J, K);
+    if (rz.cc != nullptr && ((J + K) & 1) == 0) zphi[q] = rb2_from_zero(v, rz, q, gc.sy);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-            const T* __restrict__ cc, Geo gf, Geo gc, T* __restrict__ zphi)
+            const T* __restrict__ cc, Geo gf, Geo gc, T* __restrict__ zphi, RedZ<T> rz)
 {
     const int I = 2 + blockIdx.z;
     const int J = 2 + blockIdx.y * BY + threadIdx.y;
     const int K = 2 + blockIdx.x * BX + threadIdx.x;
     if (J > gc.my || K > gc.mx) return;
     const long c = (long)(2 * I - 1) * gf.sz + (long)(2 * J - 1) * gf.sy + (2 * K - 1);
-    rc[(long)I * gc.sz + (long)J * gc.sy + K] = restrict3_at(phi, rhs, cx, cy, cz, cc, c, gf.sy, gf.sz);
+    const long q = (long)I * gc.sz + (long)J * gc.sy + K;
+    const T v = restrict3_at(phi, rhs, cx, cy, cz, cc, c, gf.sy, gf.sz);
+    rc[q] = v;
     if (zphi != nullptr) zero_with_rim(zphi, gc, 0, 3, I, J, K);
+    if (rz.cc != nullptr && ((I + J + K) & 1) == 1) zphi[q] = rb3_from_zero(v, rz, q, gc.sy, gc.sz);
 }
 
 // Two-pass restriction for large levels: pass 1 stores the residual at every
@@ -911,7 +1019,8 @@ k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
 
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
-k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, T* __restrict__ zphi)
+k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, T* __restrict__ zphi,
+                   RedZ<T> rz)
 {
     const int I = 2 + blockIdx.z;
     const int J = 2 + blockIdx.y * BY + threadIdx.y;
@@ -932,8 +1041,11 @@ k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, 
     e = e + R(1, -1, 0);
     e = e + R(-1, 1, 0);
     e = e + R(-1, -1, 0);
-    rc[(long)I * gc.sz + (long)J * gc.sy + K] = (T)0.03125 * e + (T)0.125 * R(0, 0, 0);
+    const long q = (long)I * gc.sz + (long)J * gc.sy + K;
+    const T v = (T)0.03125 * e + (T)0.125 * R(0, 0, 0);
+    rc[q] = v;
     if (zphi != nullptr) zero_with_rim(zphi, gc, 0, 3, I, J, K);
+    if (rz.cc != nullptr && ((I + J + K) & 1) == 1) zphi[q] = rb3_from_zero(v, rz, q, gc.sy, gc.sz);
 }
 
 template <typename T>
@@ -955,7 +1067,7 @@ k_resid_first2(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_restrict_gather2(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc,
-                   const int* __restrict__ act, T* __restrict__ zphi)
+                   const int* __restrict__ act, T* __restrict__ zphi, RedZ<T> rz)
 {
     const int b = blockIdx.z;
     if (act != nullptr && act[b] == 0) return;
@@ -967,8 +1079,11 @@ k_restrict_gather2(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc,
     T s = r[c + sy + 1] + r[c + sy - 1];
     s = s + r[c - sy + 1];
     s = s + r[c - sy - 1];
-    rc[(long)b * gc.sz + (long)J * gc.sy + K] = (T)0.0625 * s + (T)0.25 * r[c];
+    const long q = (long)b * gc.sz + (long)J * gc.sy + K;
+    const T v = (T)0.0625 * s + (T)0.25 * r[c];
+    rc[q] = v;
     if (zphi != nullptr) zero_with_rim(zphi, gc, (long)b * gc.sz, 2, 0, J, K);
+    if (rz.cc != nullptr && ((J + K) & 1) == 0) zphi[q] = rb2_from_zero(v, rz, q, gc.sy);
 }
 
 // ---------------------------------------------------------------------------

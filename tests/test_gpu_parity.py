@@ -280,6 +280,60 @@ def test_cycle_coarse_program_bitexact(case, cyc, coarse_m, lds):
         np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"call {rep}")
 
 
+HALVES_CASES = [c for c in CASES if c[0] in ("2d_spatial_16x32", "2d_scalar", "3d_spatial_16x8x32",
+                                               "3d_scalar_lamarr")] + [
+    ("2d_spatial_128x64", 2, (128, 64), True, True, False),
+    ("3d_spatial_32", 3, (32, 32, 32), True, True, False),
+]
+
+
+@pytest.mark.parametrize("fuse", [1, 0])
+@pytest.mark.parametrize("coarse_m", [0, None])
+@pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4], CYCLES[5]], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", HALVES_CASES, ids=[c[0] for c in HALVES_CASES])
+def test_cycle_fused_halves_bitexact(case, cyc, coarse_m, fuse):
+    """Levels smoothed one launch per half: the restriction running the
+    coarse level's first half (its phi is zero) and the prolongation applied
+    inside the first post-smoothing half (JSV_TUNE_FUSE_HALVES on and off,
+    every level launched or the default coarse program) give the oracle's
+    bits for whole cycles."""
+    from jsolver_amd import _lib
+    lib = _lib.load()
+    dp, op, phi, rhs = _setup(case, seed=23)
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_HALVES, fuse))
+    if coarse_m is not None:
+        _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, coarse_m))
+    kw = cyc[1]
+    c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+    want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
+    for rep in range(2):
+        dp.upload(0, 0, phi)
+        dp.upload(0, 1, rhs)
+        dp.cycle(cfg(**kw))
+        np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"call {rep}")
+
+
+@pytest.mark.parametrize("dim,ms", [(3, (64, 32, 32)), (2, (256, 128))])
+def test_cycle_fused_halves_fp32_matches_separate(dim, ms):
+    """FP32 cycles (the mixed solver's inner cycles): the fused halves give the
+    bits of the separate launches."""
+    from jsolver_amd import _lib
+    lib = _lib.load()
+    phi, rhs, lam, D, _ = rand_problem(dim, ms, seed=41)
+    outs = []
+    for fuse in (1, 0):
+        dp = DevPlan(dim, ms, lam, D, precision="mixed")
+        _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_HALVES, fuse))
+        _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, 0))
+        dp.upload(0, 0, gmg.zero_rim(phi).astype(np.float32), prec=_lib.JSV_MIXED)
+        dp.upload(0, 1, rhs.astype(np.float32), prec=_lib.JSV_MIXED)
+        for kw in (dict(), dict(gamma=2, nu2=2), dict(fcycle=True)):
+            dp.cycle(cfg(**kw), prec=_lib.JSV_MIXED)
+        outs.append(dp.download(0, 0, prec=_lib.JSV_MIXED))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    assert np.abs(outs[0]).max() > 0
+
+
 @pytest.mark.parametrize("solo,threads", [(0, 1024), (64, 256), (64, 512), (16, 1024)])
 @pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4]], ids=lambda c: c[0])
 @pytest.mark.parametrize("case", COARSE_CASES, ids=[c[0] for c in COARSE_CASES])
