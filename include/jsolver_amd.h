@@ -221,6 +221,17 @@ int jsv_do_bcs_3d(double *arr, int nz, int ny, int nx, int rim, int direction,
                   const double *vals_xb, const double *vals_xe,
                   const double *vals_yb, const double *vals_ye, void *stream);
 
+/* Non-zero Dirichlet values: the values phi_b holds on the level-0 boundary
+ * nodes (index 1 and m+1 per axis, as do_BCs(arr, vals_...) sets them,
+ * boundary_handler_2d.py:3-29) moved to the right-hand side of the unknowns
+ * next to them: rhs_out = rhs - sum over boundary-node neighbours q of
+ * A(p, q) phi_b[q] (the operator set on the plan; D_xy corners included).
+ * A zero-boundary solve of rhs_out is the solve with those boundary values;
+ * the reference's solver has no such option (it calls do_BCs with zeros,
+ * README.md:90).  Arrays of the plan's fine shape (batched 2D: the batch). */
+int jsv_dirichlet_lift(jsv_plan *plan, const double *phi_b, const double *rhs, double *rhs_out,
+                       void *stream);
+
 /* ---- reverse pass of fixed-mode solves (FP64, 5/7-point operators) ----
  * Replaces the reference's jax.vjp over a fixed solve
  * (examples/benchmark.py:1184-1226; tests/test_solver_2d.py:200-207).

@@ -91,6 +91,7 @@ SIGNATURES = {
     "jsv_cycle": (_I, [_P, _I, ctypes.POINTER(JsvConfig), _I, _P]),
     "jsv_residual": (_I, [_P, _P, _P, _P, _DP, _P]),
     "jsv_max_abs_diff": (_I, [_P, _P, _I64, _DP, _P]),
+    "jsv_dirichlet_lift": (_I, [_P, _P, _P, _P, _P]),
     "jsv_do_bcs_2d": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "jsv_do_bcs_3d": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "jsv_forward_checkpoints": (_I, [_P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P]),

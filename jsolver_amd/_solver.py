@@ -183,7 +183,7 @@ def validate_inputs(dim, phi, rhs, lam, D, D_xy):
 def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12, gamma=1,
           fcycle=False, skip_pre=False, nu1=1, nu2=1, max_iters_outer=INT32_MAX,
           max_iters_inner=INT32_MAX, precision="fp64", stop="distance", rtol=1e-10,
-          inner_cycles=0, check_every=1, return_history=False, return_info=False):
+          inner_cycles=0, check_every=1, return_history=False, return_info=False, boundary="zero"):
     ms = [g.mx for g in grids]
     levels, m_level = level_layout(ms)                  # Solver.__init__ -> compute_shapes
     check_grids(grids)                                  # compute_idel
@@ -192,6 +192,8 @@ def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
     if stop not in ("distance", "rel_residual"):
         raise ValueError(f"stop must be 'distance' or 'rel_residual', got {stop!r}")
+    if boundary not in ("zero", "phi"):
+        raise ValueError(f"boundary must be 'zero' or 'phi', got {boundary!r}")
     # max_iters_inner only feeds the reference's unused coarsest_level_solve
     # (solver_2d.py:547-580); it is accepted and has no effect, as there.
     del max_iters_inner
@@ -202,6 +204,10 @@ def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12
     dev = device()
     args = [phi, rhs, lam, *D] + ([D_xy] if D_xy is not None else [])
     from . import tangent
+    differentiated = any(tangent.has_tangent(v) for v in args) or (
+        torch.is_grad_enabled() and any(torch.is_tensor(v) and v.requires_grad for v in args))
+    if boundary == "phi" and differentiated:
+        raise NotImplementedError("boundary='phi' is not differentiable; use zero boundaries")
     if any(tangent.has_tangent(v) for v in args):
         if return_history:
             raise ValueError("return_history is not available for differentiated solves")
@@ -231,6 +237,8 @@ def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12
         D_v = [scalar_value(d) for d in D]
         Dxy_v = None
     plan.set_operator(lam_v, D_v, Dxy_v)
+    if boundary == "phi":
+        rhs_d = dirichlet_lift(plan, phi_d, rhs_d)
     hist = None
     if return_history:
         cap = max_iters_outer if (fixed or max_iters_outer < 100000) else 100000
@@ -242,6 +250,8 @@ def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12
     _lib.check(_lib.load().jsv_solve(plan.h, ptr(phi_d), ptr(rhs_d), ptr(out), ctypes.byref(cfg),
                                      ctypes.byref(res), stream_ptr()))
     count = int(res.count)
+    if boundary == "phi":
+        out = with_boundary(out, phi_d, dim)
     ret = (out, count)
     if return_history:
         n = int(res.outer_steps) if precision == "mixed" else count
@@ -251,6 +261,24 @@ def solve(dim, grids, phi, rhs, lam, D, D_xy=None, *, fixed=False, epsilon=1e-12
                       "distance": float(res.distance), "rel_residual": float(res.rel_residual),
                       "nonfinite": bool(res.nonfinite), "levels": levels},)
     return ret
+
+
+def dirichlet_lift(plan, phi_b, rhs):
+    """rhs with the Dirichlet values phi_b holds on the boundary nodes moved to
+    the neighbouring unknowns (jsv_dirichlet_lift; the plan's operator)."""
+    out = torch.empty_like(rhs)
+    _lib.check(_lib.load().jsv_dirichlet_lift(plan.h, ptr(phi_b), ptr(rhs), ptr(out), stream_ptr()))
+    return out
+
+
+def with_boundary(out, phi_b, dim, batched=False):
+    """The solution on the unknowns, phi_b's values (boundary nodes and
+    ghosts) everywhere else."""
+    inner = (slice(None),) if batched else ()
+    inner = inner + (slice(2, -2),) * dim
+    res = phi_b.clone()
+    res[inner] = out[inner]
+    return res
 
 
 def _solve_differentiable(dim, plan, dev, full_shape, phi, rhs, lam, D, D_xy, fixed, precision,
@@ -295,7 +323,7 @@ def _solve_differentiable(dim, plan, dev, full_shape, phi, rhs, lam, D, D_xy, fi
 def solve_batched_2d(grid_x, grid_y, phi, rhs, lam, D_xx, D_yy, *, D_xy=None, fixed=False,
                      epsilon=1e-12, gamma=1, fcycle=False, skip_pre=False, nu1=1, nu2=1,
                      max_iters_outer=INT32_MAX, precision="fp64", stop="distance", rtol=1e-10,
-                     inner_cycles=0, check_every=1):
+                     inner_cycles=0, check_every=1, boundary="zero"):
     """B independent 2D problems on one grid, solved together (one launch per
     operation for the whole batch).  phi, rhs and array coefficients have shape
     (B, my+3, mx+3); lam / D may be Python floats shared by all problems.
@@ -327,6 +355,10 @@ def solve_batched_2d(grid_x, grid_y, phi, rhs, lam, D_xx, D_yy, *, D_xy=None, fi
         raise ValueError("D_xx and D_yy must both be scalars or both be arrays")
     Dxy_v = arr_or_scalar(D_xy) if D_xy is not None else None
     plan.set_operator(lam_v, D_v, Dxy_v)
+    if boundary not in ("zero", "phi"):
+        raise ValueError(f"boundary must be 'zero' or 'phi', got {boundary!r}")
+    if boundary == "phi":
+        rhs_d = dirichlet_lift(plan, phi_d, rhs_d)
     cfg = make_config(fixed, epsilon, gamma, fcycle, skip_pre, nu1, nu2, max_iters_outer, stop, rtol,
                       inner_cycles, check_every=check_every)
     out = torch.empty_like(phi_d)
@@ -337,4 +369,6 @@ def solve_batched_2d(grid_x, grid_y, phi, rhs, lam, D_xx, D_yy, *, D_xy=None, fi
         plan.h, ptr(phi_d), ptr(rhs_d), ptr(out), ctypes.byref(cfg), ctypes.byref(res),
         counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
         stopv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), stream_ptr()))
+    if boundary == "phi":
+        out = with_boundary(out, phi_d, 2, batched=True)
     return out, counts, stopv

@@ -1565,6 +1565,20 @@ int jsv_residual(jsv_plan* P, const double* phi, const double* rhs, double* out_
     return 0;
 }
 
+int jsv_dirichlet_lift(jsv_plan* P, const double* phi_b, const double* rhs, double* rhs_out, void* stream)
+{
+    if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
+    if (phi_b == nullptr || rhs == nullptr || rhs_out == nullptr) return fail("NULL argument");
+    const Hier<double>& H = P->h64;
+    const Geo& g = H.g[0];
+    const bool od = P->dim == 2 && P->offdiag;
+    hipLaunchKernelGGL(jsv::k_dirichlet_lift, grid_exact(g.n), dim3(256), 0, (hipStream_t)stream, rhs_out, rhs,
+                       phi_b, H.cx[0], H.cy[0], P->dim == 3 ? H.cz[0] : nullptr, od ? H.drr[0] : nullptr,
+                       od ? H.dlr[0] : nullptr, g, P->dim);
+    LAUNCHCHK();
+    return 0;
+}
+
 int jsv_max_abs_diff(const double* a, const double* b, int64_t n, double* out, void* stream)
 {
     if (a == nullptr || b == nullptr || out == nullptr) return fail("NULL argument");

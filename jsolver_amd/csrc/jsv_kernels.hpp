@@ -1879,4 +1879,48 @@ k_setup3d(double* __restrict__ cx, double* __restrict__ cy, double* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Non-zero Dirichlet values (the reference's do_BCs with vals_*,
+// boundary_handler_2d.py:3-29, which its solver always calls with zeros,
+// README.md:90): the values pb holds on the boundary nodes (index 1 and m+1
+// per axis) are moved to the right-hand side of the level-0 unknowns next to
+// them, rhs_out = rhs - sum over boundary-node neighbours q of A(p, q) pb[q],
+// so the zero-boundary solve of rhs_out gives the solution with those values
+// (the smoother and the residual see the same sums).  Order per point: x-,
+// x+, y-, y+, z-, z+, then the D_xy corners (y-1,x-1), (y-1,x+1), (y+1,x-1),
+// (y+1,x+1).  Other entries copy rhs.  2D: z indexes the problems of a batch.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_dirichlet_lift(double* __restrict__ out, const double* __restrict__ rhs, const double* __restrict__ pb,
+                 const double* __restrict__ cx, const double* __restrict__ cy, const double* __restrict__ cz,
+                 const double* __restrict__ drr, const double* __restrict__ dlr, Geo g, int dim)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.n) return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = dim == 3 ? (int)(i / g.sz) : 0;
+    double v = rhs[i];
+    const bool unk = x >= 2 && x <= g.mx && y >= 2 && y <= g.my && (dim == 2 || (z >= 2 && z <= g.mz));
+    if (unk) {
+        const long sy = g.sy, sz = g.sz;
+        auto bx = [&](int xx) { return xx == 1 || xx == g.mx + 1; };
+        auto by = [&](int yy) { return yy == 1 || yy == g.my + 1; };
+        if (bx(x - 1)) v = v - cx[i - 1] * pb[i - 1];
+        if (bx(x + 1)) v = v - cx[i] * pb[i + 1];
+        if (by(y - 1)) v = v - cy[i - sy] * pb[i - sy];
+        if (by(y + 1)) v = v - cy[i] * pb[i + sy];
+        if (dim == 3) {
+            if (z - 1 == 1) v = v - cz[i - sz] * pb[i - sz];
+            if (z + 1 == g.mz + 1) v = v - cz[i] * pb[i + sz];
+        } else if (drr != nullptr) {
+            if (by(y - 1) || bx(x - 1)) v = v - drr[i - sy - 1] * pb[i - sy - 1];
+            if (by(y - 1) || bx(x + 1)) v = v - dlr[i - sy + 1] * pb[i - sy + 1];
+            if (by(y + 1) || bx(x - 1)) v = v - dlr[i] * pb[i + sy - 1];
+            if (by(y + 1) || bx(x + 1)) v = v - drr[i] * pb[i + sy + 1];
+        }
+    }
+    out[i] = v;
+}
+
 }  // namespace jsv

@@ -16,6 +16,12 @@ Keyword-only additions to ``solve_2d`` (no counterpart in the reference):
                          (0 = spaced by the predicted cycles-to-tolerance)
   return_history=False -- also return the per-cycle stop quantity (numpy)
   return_info=False   -- also return a dict (count, outer steps, last distance ...)
+  boundary="zero" | "phi" -- "phi": Dirichlet values are the boundary layers of
+                         the incoming phi (set them with do_BCs(phi, vals_...));
+                         the result keeps them.  The reference's solver always
+                         uses zeros (README.md:90).  rel_residual is then
+                         relative to the right-hand side with the boundary
+                         values moved into it.
 """
 import numpy as np
 

@@ -4,8 +4,8 @@ on MI355X.
 Same conventions as solver_2d: the reference's names, signatures and defaults;
 returns ``(phi, count)`` with ``phi`` a float64 torch tensor on the GPU
 (ghost-padded ``(mz+3, my+3, mx+3)``).  The keyword-only additions of
-``solve_2d`` (precision, stop, rtol, inner_cycles, return_history,
-return_info) apply here too.  There is no D_xy in 3D.
+``solve_2d`` (precision, stop, rtol, inner_cycles, check_every, return_history,
+return_info, boundary) apply here too.  There is no D_xy in 3D.
 """
 import numpy as np
 

@@ -541,6 +541,62 @@ def residual(op: Operator, phi: np.ndarray, rhs: np.ndarray, level: int = 0) -> 
     return zero_rim(out)
 
 
+def dirichlet_lift(op: Operator, phi_b: np.ndarray, rhs: np.ndarray) -> np.ndarray:
+    """Non-zero Dirichlet values (an extension: the reference's solver always
+    calls do_BCs with zeros, README.md:90; do_BCs(arr, vals_...) is
+    boundary_handler_2d.py:3-29).  The values phi_b holds on the level-0
+    boundary nodes (index 1 and m+1 per axis) move to the right-hand side of
+    the unknowns next to them: rhs - sum over boundary-node neighbours q of
+    A(p, q) phi_b[q], with A the smoother's neighbour coefficients (face
+    coefficients and D_xy corners).  A zero-boundary solve of the result
+    equals a solve whose boundary nodes hold phi_b, sweep by sweep up to
+    rounding.  Subtraction order per point: axes fastest first (x, y[, z]),
+    minus side then plus side, then the corners ll, rl, lr, rr."""
+    d = op.dim
+    lv = op.lv[0]
+    out = np.array(rhs, dtype=np.result_type(_ft(rhs, phi_b), op.dtype), copy=True)
+    shape = rhs.shape
+    m = [n - 3 for n in shape]                 # per array axis
+    inner = tuple(slice(2, n - 2) for n in shape)
+    grids = np.meshgrid(*[np.arange(2, n - 2) for n in shape], indexing="ij")
+
+    def bnd(a, idx):
+        return (idx == 1) | (idx == m[a] + 1)
+
+    def coef(arr, offs):
+        # coefficient array on the [1:-1] window, at unknown + offs (window index = array index - 1)
+        if np.ndim(arr) == 0:
+            return arr
+        sl = tuple(slice(1 + o, n - 3 + o) for o, n in zip(offs, shape))
+        return arr[sl]
+
+    def nbval(offs):
+        sl = tuple(slice(2 + o, n - 2 + o) for o, n in zip(offs, shape))
+        return phi_b[sl]
+
+    def sub(mask, c, v):
+        o = out[inner]
+        o[mask] = o[mask] - (c * v)[mask] if np.ndim(c) else o[mask] - (c * v[mask])
+        out[inner] = o
+
+    for a in reversed(range(d)):
+        for sgn in (-1, +1):
+            offs = [0] * d
+            offs[a] = sgn
+            mask = bnd(a, grids[a] + sgn)
+            if lv.spatial:
+                c = coef(lv.a_minus[a] if sgn < 0 else lv.a_plus[a], [0] * d)
+            else:
+                c = lv.a_axis[a]
+            sub(mask, c, nbval(offs))
+    if op.off_diag:
+        dg = lv.a_diag
+        for key, (oy, ox) in (("ll", (-1, -1)), ("rl", (-1, 1)), ("lr", (1, -1)), ("rr", (1, 1))):
+            mask = bnd(0, grids[0] + oy) | bnd(1, grids[1] + ox)
+            sub(mask, coef(dg[key], [0, 0]), nbval([oy, ox]))
+    return out
+
+
 def residual_norm(op: Operator, phi, rhs) -> float:
     return float(np.linalg.norm(residual(op, phi, rhs)))
 

@@ -243,3 +243,76 @@ def test_ghost_values_have_no_effect(dim, ms):
                                                        fixed=True, max_iters_outer=4)[0]
     inner = tuple(slice(1, n - 1) for n in shape)
     np.testing.assert_array_equal(got[inner], base[inner])
+
+
+# ---------------------------------------------------------------------------
+# non-zero Dirichlet values (gmg.dirichlet_lift): the converged multigrid
+# solution of the lifted right-hand side equals a direct solve of the level-0
+# system whose boundary nodes hold the values, with the boundary couplings
+# taken from the same independent face rules as assemble_level0
+# ---------------------------------------------------------------------------
+def _boundary_rhs(dim, ms, D, g, rhs, Dxy=None):
+    shape = tuple(m + 3 for m in reversed(ms))
+    idel = [gg.idel for gg in _grids(ms)]
+    inner = [np.arange(2, n - 2) for n in shape]
+    mesh = np.meshgrid(*inner, indexing="ij")
+    Darr = [np.broadcast_to(np.asarray(d, dtype=float), shape) for d in D]
+    b = rhs[tuple(mesh)].astype(float).copy()
+    it = np.nditer(mesh[0], flags=["multi_index"])
+    for _ in it:
+        p = tuple(mm[it.multi_index] for mm in mesh)
+        k = it.multi_index
+        for a in range(dim):
+            phys = dim - 1 - a
+            s_ = idel[phys] ** 2
+            Dp = Darr[phys]
+            for sgn in (+1, -1):
+                q = list(p)
+                q[a] += sgn
+                q = tuple(q)
+                if not (q[a] == 1 or q[a] == shape[a] - 2):
+                    continue
+                lo, hi = (p, q) if sgn > 0 else (q, p)
+                f = 0.5 * (Dp[lo] + Dp[hi]) * s_ if dim == 2 else Dp[lo] * s_
+                b[k] -= f * g[q]
+        if Dxy is not None:
+            c = idel[0] * idel[1] * 0.25
+            y, x = p
+            Q = Dxy
+            terms = {(-1, -1): (Q[y, x - 1] + Q[y - 1, x]) * c,
+                     (+1, -1): -(Q[y, x - 1] + Q[y + 1, x]) * c,
+                     (-1, +1): -(Q[y, x + 1] + Q[y - 1, x]) * c,
+                     (+1, +1): (Q[y, x + 1] + Q[y + 1, x]) * c}
+            for (dy, dx), v in terms.items():
+                q = (y + dy, x + dx)
+                if q[0] in (1, shape[0] - 2) or q[1] in (1, shape[1] - 2):
+                    b[k] -= v * g[q]
+    return b, mesh
+
+
+@pytest.mark.parametrize("dim,ms,dxy,spatial", [(2, (8, 8), False, True), (2, (16, 8), True, True),
+                                                (2, (8, 16), False, False), (3, (8, 8, 8), False, True),
+                                                (3, (8, 16, 8), False, False)])
+def test_dirichlet_lift_matches_direct_solve(dim, ms, dxy, spatial):
+    rng = np.random.default_rng(11)
+    shape = tuple(m + 3 for m in reversed(ms))
+    D = [np.exp(0.3 * rng.standard_normal(shape)) for _ in range(dim)] if spatial else [1.3, 0.7, 2.0][:dim]
+    lam = rng.uniform(0, 1, shape)
+    rhs = rng.standard_normal(shape)
+    Dxy = 0.1 * rng.standard_normal(shape) if dxy else None
+    g = rng.standard_normal(shape)
+    A, _, _ = assemble_level0(dim, ms, lam, D, Dxy)
+    b, mesh = _boundary_rhs(dim, ms, D, g, rhs, Dxy)
+    x = spla.spsolve(A.tocsc(), b.ravel()).reshape(mesh[0].shape)
+    op = gmg.build_operator(dim, ms, _idels(ms), lam, list(D), Dxy)
+    phi, _ = gmg.solve(op, gmg.Config(fixed=True, max_iters_outer=300), np.zeros(shape),
+                       gmg.dirichlet_lift(op, g, rhs))
+    got = phi[tuple(mesh)]
+    assert np.max(np.abs(got - x)) <= 1e-10 * np.max(np.abs(x))
+    # a lift that dropped a term (here the last one applied) fails the pin
+    if dim == 2 and not dxy:
+        bad = rhs.copy()
+        rl = gmg.dirichlet_lift(op, g, rhs)
+        bad[2:-2, -3] = rl[2:-2, -3] + 0.5 * (rl[2:-2, -3] - rhs[2:-2, -3])
+        pb, _ = gmg.solve(op, gmg.Config(fixed=True, max_iters_outer=300), np.zeros(shape), bad)
+        assert np.max(np.abs(pb[tuple(mesh)] - x)) > 1e-3 * np.max(np.abs(x))
