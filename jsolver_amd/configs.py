@@ -14,7 +14,9 @@ configuration, on one GPU:
    mixed (the V-cycle case is bench.py's).
 4. 2D 512^2 anisotropic (r = 10), 8 fixed V-cycles, L = 0.5 ||phi - phi_t||^2:
    time of the forward solve and of the hand-written adjoint, their ratio,
-   and the peak device memory of each.
+   and the peak device memory of each; the forward-mode directional
+   derivative of L (tangent cycles), its time, and its agreement with the
+   adjoint gradient.
 """
 import argparse
 import json
@@ -148,6 +150,13 @@ def config4():
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
     mem_all = torch.cuda.max_memory_allocated()
+    # forward mode: directional derivative of L along seeded directions of
+    # (rhs, lambda, D_xx, D_yy) by the tangent cycles, against <grad L, v>
+    gen = torch.Generator(device="cpu").manual_seed(4)
+    dirs = [torch.randn(b.shape, generator=gen, dtype=torch.float64).to(dev) for b in base]
+    f = lambda *a: forward(list(a))
+    (_, dL), t_jvp = _timed(lambda: torch.func.jvp(f, tuple(base), tuple(dirs)), reps=3)
+    dL_rev = float(sum(torch.sum(x.grad * v) for x, v in zip(ts, dirs)))
     plan = _solver.get_plan(2, (m, m), (gx.idel, gy.idel), "fp64")
     return {"config": "2D 512^2 anisotropic r=10, 8 fixed V-cycles, adjoint of L = 0.5||phi - phi_t||^2",
             "loss": float(L.detach()), "forward_s": round(t_fwd, 5), "forward_plain_s": round(t_plain, 5),
@@ -155,6 +164,9 @@ def config4():
             "backward_over_plain_forward": round(t_bwd / t_plain, 3),
             "peak_torch_bytes_forward": int(mem_fwd), "peak_torch_bytes_with_backward": int(mem_all),
             "plan_device_bytes": plan.device_bytes(), "adjoint_device_bytes": adjoint.adjoint_device_bytes(plan),
+            "jvp_s": round(t_jvp, 5), "jvp_over_plain_forward": round(t_jvp / t_plain, 3),
+            "jvp_directional_derivative": float(dL),
+            "jvp_vs_adjoint_rel_diff": abs(float(dL) - dL_rev) / abs(dL_rev),
             "note": "torch peaks count inputs, the per-cycle checkpoints and the gradients; the plan's level "
                     "buffers and the adjoint's work arrays are the library's own allocations, listed apart"}
 

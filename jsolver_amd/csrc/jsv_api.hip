@@ -528,7 +528,8 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
                                H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr(), zphi, rz);
         } else {
             hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
-                               H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf);
+                               H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf,
+                               gf.n < (1L << 24) ? 1 : 0);
             hipLaunchKernelGGL(jsv::k_restrict_gather3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
                                H.rhs[l + 1], H.res[l], gf, gc, zphi, rz);
         }

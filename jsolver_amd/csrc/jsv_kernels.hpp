@@ -34,6 +34,27 @@ struct Geo {
 constexpr int BX = 64;   // one wave across x
 constexpr int BY = 4;    // 4 waves per workgroup
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
+// remap the launch index so XCD k works on a contiguous run of tiles and the
+// halo rows neighbouring tiles share are fetched once per L2.
+__device__ __forceinline__ int xcd_tile(int id, int n)
+{
+    const int per = n >> 3;
+    return id < (per << 3) ? (id & 7) * per + (id >> 3) : id;
+}
+
+// The launch's block index remapped the same way for the one-point-per-thread
+// 3D kernels: XCD k works on the k-th contiguous eighth of the row-major
+// block order (a z-slab of the level), so the neighbouring rows and planes a
+// stencil reads mostly belong to blocks of the same XCD and are fetched once
+// into its L2.
+__device__ __forceinline__ uint3 xcd_block()
+{
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int t = xcd_tile((int)(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z)), gx * gy * (int)gridDim.z);
+    return make_uint3((unsigned)(t % gx), (unsigned)((t / gx) % gy), (unsigned)(t / (gx * gy)));
+}
+
 // ---------------------------------------------------------------------------
 // Red-black Gauss-Seidel half sweeps (solver_2d.py:670-692, solver_3d.py:679-706).
 // The reference's sub-colours split into two independent halves: in 2D the
@@ -132,10 +153,11 @@ k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
       const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc, Geo g, int par,
       int z0)
 {
-    const int y = 2 + blockIdx.y * BY + threadIdx.y;
-    const int z = z0 + blockIdx.z;
+    const uint3 bk = xcd_block();
+    const int y = 2 + bk.y * BY + threadIdx.y;
+    const int z = z0 + bk.z;
     if (y > g.my) return;
-    const int x = 2 + ((par + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    const int x = 2 + ((par + y + z) & 1) + 2 * (bk.x * BX + threadIdx.x);
     if (x > g.mx) return;
     rb3_point(phi, rhs, cx, cy, cz, cc, (long)z * g.sz + (long)y * g.sy + x, g.sy, g.sz);
 }
@@ -263,10 +285,11 @@ k_rb3_pro(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ 
           const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc, Geo g, int par,
           const T* __restrict__ pc, Geo gc)
 {
-    const int y = 2 + blockIdx.y * BY + threadIdx.y;
-    const int z = 2 + blockIdx.z;
+    const uint3 bk = xcd_block();
+    const int y = 2 + bk.y * BY + threadIdx.y;
+    const int z = 2 + bk.z;
     if (y > g.my) return;
-    const int x = 2 + ((par + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    const int x = 2 + ((par + y + z) & 1) + 2 * (bk.x * BX + threadIdx.x);
     if (x > g.mx) return;
     const long sy = g.sy, sz = g.sz;
     const long p = (long)z * sz + (long)y * sy + x;
@@ -333,15 +356,6 @@ k_rb2_pro(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ 
 // Rim values of phi_out are never written (zero from allocation).
 // ---------------------------------------------------------------------------
 extern __shared__ double jsv_dyn_smem[];
-
-// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
-// remap the launch index so XCD k works on a contiguous run of tiles and the
-// halo rows neighbouring tiles share are fetched once per L2.
-__device__ __forceinline__ int xcd_tile(int id, int n)
-{
-    const int per = n >> 3;
-    return id < (per << 3) ? (id & 7) * per + (id >> 3) : id;
-}
 
 template <typename T, int TY, int NT>
 __global__ void __launch_bounds__(NT)
@@ -1006,12 +1020,15 @@ template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
                const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-               const T* __restrict__ cc, Geo g)
+               const T* __restrict__ cc, Geo g, int remap)
 {
-    const int y = 2 + blockIdx.y * BY + threadIdx.y;
-    const int z = 2 + blockIdx.z;
+    // remap: XCD-slab block order (pays on levels the Infinity Cache holds;
+    // on larger levels the launch order streams HBM better)
+    const uint3 bk = remap ? xcd_block() : make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
+    const int y = 2 + bk.y * BY + threadIdx.y;
+    const int z = 2 + bk.z;
     if (y > g.my) return;
-    const int x = 2 + ((1 + y + z) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    const int x = 2 + ((1 + y + z) & 1) + 2 * (bk.x * BX + threadIdx.x);
     if (x > g.mx) return;
     const long p = (long)z * g.sz + (long)y * g.sy + x;
     r[res3_at(g, z, y, x)] = resid3<T, T>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);
@@ -1022,9 +1039,10 @@ __global__ void __launch_bounds__(BX * BY)
 k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, T* __restrict__ zphi,
                    RedZ<T> rz)
 {
-    const int I = 2 + blockIdx.z;
-    const int J = 2 + blockIdx.y * BY + threadIdx.y;
-    const int K = 2 + blockIdx.x * BX + threadIdx.x;
+    const uint3 bk = xcd_block();
+    const int I = 2 + bk.z;
+    const int J = 2 + bk.y * BY + threadIdx.y;
+    const int K = 2 + bk.x * BX + threadIdx.x;
     if (J > gc.my || K > gc.mx) return;
     // residual at fine offset (dz, dy, dx) from the centre (2I-1, 2J-1, 2K-1)
     auto R = [&](int dz, int dy, int dx) -> T {
