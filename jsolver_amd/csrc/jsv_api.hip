@@ -527,9 +527,15 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
             hipLaunchKernelGGL(jsv::k_restrict_gather2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
                                H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr(), zphi, rz);
         } else {
-            hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
-                               H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf,
-                               gf.n < (1L << 24) ? 1 : 0);
+            if (gf.n < (1L << 24)) {
+                hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
+                                   H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 1);
+            } else {
+                constexpr int RB = 16;
+                const dim3 gt(gf_half.x, (unsigned)((gf.my - 1 + RB - 1) / RB), gf_half.z);
+                hipLaunchKernelGGL((jsv::k_resid_first3<T, RB>), gt, dim3(jsv::BX, RB), 0, s, H.res[l],
+                                   H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 0);
+            }
             hipLaunchKernelGGL(jsv::k_restrict_gather3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
                                H.rhs[l + 1], H.res[l], gf, gc, zphi, rz);
         }

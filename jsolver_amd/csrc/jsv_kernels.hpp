@@ -1016,8 +1016,10 @@ __device__ __forceinline__ long res3_at(const Geo& g, int z, int y, int x)
     return ((long)(z - 2) * (g.my - 1) + (y - 2)) * (g.mx >> 1) + ((x - 2) >> 1);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BX * BY)
+// RB rows per block: tall blocks on large levels, so the rows above and below
+// a block (read for the y-neighbours) are few against the rows it owns.
+template <typename T, int RB = BY>
+__global__ void __launch_bounds__(BX * RB)
 k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
                const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
                const T* __restrict__ cc, Geo g, int remap)
@@ -1025,7 +1027,7 @@ k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
     // remap: XCD-slab block order (pays on levels the Infinity Cache holds;
     // on larger levels the launch order streams HBM better)
     const uint3 bk = remap ? xcd_block() : make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
-    const int y = 2 + bk.y * BY + threadIdx.y;
+    const int y = 2 + bk.y * RB + threadIdx.y;
     const int z = 2 + bk.z;
     if (y > g.my) return;
     const int x = 2 + ((1 + y + z) & 1) + 2 * (bk.x * BX + threadIdx.x);
