@@ -238,10 +238,14 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
 // LDS of the pair sweep with TY rows and np pairs per thread (rings + held
 // black halves, plus the coarse-plane ring when the prolongation from
 // coarse level gc is fused)
+// ring row width of the pair kernel instantiated for level g (its NHMAX)
+int pair_nhm(const Geo& g) { return (g.nx + 1) / 2 <= 130 ? 130 : 258; }
+
 size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1)
 {
-    size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * ((g.nx + 1) / 2) + (size_t)8 * np * 1024;
-    if (gc != nullptr) e += (size_t)3 * (ty / 2 + 4) * gc->nx;
+    const int nhm = pair_nhm(g);
+    size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * nhm + (size_t)8 * np * 1024;
+    if (gc != nullptr) e += (size_t)3 * (ty / 2 + 4) * (nhm + 1);
     return w * e;
 }
 
@@ -297,13 +301,26 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
     const int ytiles = (g.my - 1 + TY - 1) / TY;
     const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
     const dim3 grid((unsigned)(ytiles * zchunks));
+    // a row of pairs fills whole waves: the kernel variant with the pair
+    // element picks resolved per wave at compile time, unless that variant
+    // needs scratch memory (register spills cost more than the picks save)
+    auto no_scratch = [](const void* k) {
+        hipFuncAttributes a;
+        return hipFuncGetAttributes(&a, k) == hipSuccess && a.localSizeBytes == 0;
+    };
+    static const bool wr_pro = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP, 1, true>);
+    static const bool wr_plain = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true>);
+    const bool wr = ((g.mx / 2) / NP) % 64 == 0 && (pc != nullptr ? wr_pro : wr_plain);
+    auto go = [&](auto kern, const T* pcv) {
+        hipLaunchKernelGGL(kern, grid, dim3(NT), lds, s, H.phi[l], H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l],
+                           H.cc[l], g, zchunk, ytiles, pcv, gc);
+    };
     if (pc != nullptr)
-        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP>), grid, dim3(NT), lds, s, H.phi[l],
-                           H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pc, gc);
+        wr ? go(jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP, 1, true>, pc)
+           : go(jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP, 1, false>, pc);
     else
-        hipLaunchKernelGGL((jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP>), grid, dim3(NT), lds, s, H.phi[l],
-                           H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles,
-                           (const T*)nullptr, gc);
+        wr ? go(jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true>, (const T*)nullptr)
+           : go(jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, false>, (const T*)nullptr);
     return 0;
 }
 

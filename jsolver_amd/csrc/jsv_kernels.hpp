@@ -506,6 +506,16 @@ struct __attribute__((aligned(sizeof(T)))) PairVec {
     T v[N];
 };
 
+// which element of a column pair is red on a plane: a compile-time constant
+// (PairElem) or computed from y + z (PairElemRT)
+template <int V>
+struct PairElem {
+    __device__ static constexpr int at(int, int) { return V; }
+};
+struct PairElemRT {
+    __device__ static int at(int y, int z) { return ((y + z) & 1) != 0 ? 0 : 1; }
+};
+
 // the coefficients of NP consecutive column pairs of one row and plane
 template <typename T, int NP>
 struct PairGrp {
@@ -513,7 +523,7 @@ struct PairGrp {
     PairVec<T, 2 * NP> cx, cy, cym, cz, rh, dg;   // at x = 2k .. 2k + 2NP - 1; cym: row y - 1
 };
 
-template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int NP = 1, int WPE = 1>
+template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int NP = 1, int WPE = 1, bool WR = false>
 __global__ void __launch_bounds__(NT, WPE)
 k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
@@ -524,7 +534,10 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     constexpr int PMAX = (PR * NHMAX + NT - 1) / NT;   // ring elements per thread (nh <= NHMAX)
     constexpr int CRR = TY / 2 + 4;             // coarse rows under the phi_in ring rows
     constexpr int CMAX = (CRR * (NHMAX + 1) + NT - 1) / NT;   // coarse ring elements per thread
-    const int nh = (g.nx + 1) >> 1;
+    // ring rows are NHMAX wide (the host sizes LDS for that): constant strides
+    // fold the neighbour offsets into the LDS instructions; columns at or past
+    // (nx + 1) / 2 hold zeros and are never read
+    constexpr int nh = NHMAX;
     const int nhe = g.mx >> 1;                  // pairs per row
     const int gpr = nhe / NP;                   // thread columns per row (NP pairs each)
     T* P = reinterpret_cast<T*>(jsv_dyn_smem);  // [3][PR][nh]
@@ -568,7 +581,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         const int yy = y0 - 2 + rr_;
         const int x = 2 * kk + ((yy + z) & 1);
         ok = i < PR * nh && z >= 0 && z < g.nz && yy >= 0 && yy < g.ny && x < g.nx;
-        return ok ? (unsigned)(((long)z * sz + (long)yy * sy + x) * (long)sizeof(T)) : 0u;
+        return ok ? (unsigned)z * szb + (unsigned)yy * syb + (unsigned)x * W : 0u;
     };
     auto load_plane = [&](int z, auto corr_fn) {
         T* dst = P + (z % 3) * PR * nh;
@@ -594,7 +607,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // phi_in's black values as they enter the ring (with nu2 > 0 the red
     // points' corrections are never read).  e comes from a ring of three
     // coarse planes; fine plane f needs coarse planes (f+1)>>1 .. (f+2)>>1.
-    const int cnx = gc.nx;
+    constexpr int cnx = NHMAX + 1;              // coarse ring row width (>= gc.nx)
     const int cr0 = (y0 - 2) >> 1;              // first coarse row of the ring
     auto cval = [&](int I, int J, int K) -> T { return Cr[((I % 3) * CRR + (J - cr0)) * cnx + K]; };
     // correction of ring element i of fine plane z (false: none)
@@ -637,7 +650,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     auto csrc = [&](int I, int i, bool& ok) -> unsigned {
         const int rr_ = i / cnx, kk = i - rr_ * cnx;
         const int J = cr0 + rr_;
-        ok = i < CRR * cnx && J < gc.ny && I >= 0 && I < gc.nz;
+        ok = i < CRR * cnx && kk < gc.nx && J < gc.ny && I >= 0 && I < gc.nz;
         return ok ? (unsigned)(((long)I * gc.sz + (long)J * gc.sy + kk) * (long)sizeof(T)) : 0u;
     };
     auto cload_plane = [&](int I) {
@@ -725,7 +738,9 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
 #pragma unroll
     for (int pp = 0; pp < NP; ++pp) red_prev[pp] = (T)0;
 
-    auto step = [&](int zs, PairGrp<T, NP>& cur, PairGrp<T, NP>& nxt) {
+    // ec: which element of a pair is red on this step's plane, as a compile-time
+    // constant (see the loop below)
+    auto step = [&](int zs, PairGrp<T, NP>& cur, PairGrp<T, NP>& nxt, auto ec) {
         // unconditional issue: planes up to z1 + 2 <= nz are requested (the
         // ring's guard zeroes plane nz; coefficient plane zs + 1 <= nz - 1)
         commit(zs + 1);
@@ -743,8 +758,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         load(zs + 1, nxt);
         __syncthreads();
         // ---- red on plane zs (x + y + z odd); pair pp covers x = 2(k+pp), 2(k+pp)+1
-        const bool left = ((y + zs) & 1) != 0;  // the even x of each pair is red
-        const int e = left ? 0 : 1;             // red element within a pair
+        const int e = decltype(ec)::at(y, zs);  // red element within a pair: 0 when y + zs is odd
         // element of pair pp: e ? odd : even (selects between constant indices;
         // a runtime index would move the vectors to scratch memory)
 #define JSV_EL(vec, pp, odd) ((odd) ? (vec).v[2 * (pp) + 1] : (vec).v[2 * (pp)])
@@ -820,9 +834,30 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         // by this step's red half before the second barrier
         prep_corr(zs + 2);
     };
-    for (int zs = z0 - 1; zs <= z1; zs += 2) {
-        step(zs, A, B);
-        if (zs + 1 <= z1) step(zs + 1, B, A);
+    // The red element of a pair is 0 where y + z is odd, 1 where it is even, so
+    // it alternates with the step.  When a row of pairs is a whole number of
+    // waves (gpr a multiple of 64: every wave holds one row, or idle threads
+    // only), the branch on the row's parity is uniform within each wave and
+    // each path has the element picks resolved at compile time; both paths run
+    // the same barriers in the same order (s_barrier counts the waves of the
+    // workgroup).  Otherwise (WR false) a wave holds several rows and the
+    // element is computed per thread.  The host sets WR from gpr.
+    if constexpr (WR) {
+        const bool odd_first = ((y + z0 - 1) & 1) != 0;
+        for (int zs = z0 - 1; zs <= z1; zs += 2) {
+            if (odd_first) {
+                step(zs, A, B, PairElem<0>{});
+                if (zs + 1 <= z1) step(zs + 1, B, A, PairElem<1>{});
+            } else {
+                step(zs, A, B, PairElem<1>{});
+                if (zs + 1 <= z1) step(zs + 1, B, A, PairElem<0>{});
+            }
+        }
+    } else {
+        for (int zs = z0 - 1; zs <= z1; zs += 2) {
+            step(zs, A, B, PairElemRT{});
+            if (zs + 1 <= z1) step(zs + 1, B, A, PairElemRT{});
+        }
     }
 }
 
