@@ -843,14 +843,14 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // workgroup).  Otherwise (WR false) a wave holds several rows and the
     // element is computed per thread.  The host sets WR from gpr.
     if constexpr (WR) {
+        // the branch sits inside the loop, so the loop counter stays uniform
         const bool odd_first = ((y + z0 - 1) & 1) != 0;
         for (int zs = z0 - 1; zs <= z1; zs += 2) {
-            if (odd_first) {
-                step(zs, A, B, PairElem<0>{});
-                if (zs + 1 <= z1) step(zs + 1, B, A, PairElem<1>{});
-            } else {
-                step(zs, A, B, PairElem<1>{});
-                if (zs + 1 <= z1) step(zs + 1, B, A, PairElem<0>{});
+            if (odd_first) step(zs, A, B, PairElem<0>{});
+            else step(zs, A, B, PairElem<1>{});
+            if (zs + 1 <= z1) {
+                if (odd_first) step(zs + 1, B, A, PairElem<1>{});
+                else step(zs + 1, B, A, PairElem<0>{});
             }
         }
     } else {
