@@ -740,7 +740,10 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
 
     // ec: which element of a pair is red on this step's plane, as a compile-time
     // constant (see the loop below)
-    auto step = [&](int zs, PairGrp<T, NP>& cur, PairGrp<T, NP>& nxt, auto ec) {
+    auto step = [&](int zs_in, PairGrp<T, NP>& cur, PairGrp<T, NP>& nxt, auto ec) {
+        // the plane index is the same in every lane; from the first lane the
+        // compiler keeps it and the ring slots derived from it in scalar registers
+        const int zs = __builtin_amdgcn_readfirstlane(zs_in);
         // unconditional issue: planes up to z1 + 2 <= nz are requested (the
         // ring's guard zeroes plane nz; coefficient plane zs + 1 <= nz - 1)
         commit(zs + 1);
