@@ -168,7 +168,7 @@ def config4():
             "jvp_s": round(t_jvp, 5), "jvp_over_plain_forward": round(t_jvp / t_plain, 3),
             "jvp_directional_derivative": float(dL),
             "jvp_vs_adjoint_rel_diff": abs(float(dL) - dL_rev) / abs(dL_rev),
-            "note": "torch peaks are above what was allocated before the forward pass (inputs included): the "
+            "note": "torch peaks are above what was allocated before the forward pass (the inputs were): the "
                     "per-cycle checkpoints, the output and the gradients; the plan's level buffers and the "
                     "adjoint's work arrays are the library's own allocations, listed apart"}
 
