@@ -1304,6 +1304,74 @@ struct LvView {
     __device__ __forceinline__ long at(int y, int x) const { return (long)y * sy + x - org; }
 };
 
+// Point updates of the coarse program.  Its small ops leave one lane one
+// point, so a point's latency is the op's: every operand is loaded before any
+// arithmetic (one LDS round trip instead of several), and the arithmetic is
+// rb2_point / rb3_point / restrict2_at's own, in the same order.
+template <typename T>
+__device__ __forceinline__ void rb2_point_pre(T* __restrict__ phi, const T* __restrict__ rhs,
+                                              const T* __restrict__ cx, const T* __restrict__ cy,
+                                              const T* __restrict__ cc, long p, long sy)
+{
+    const T a0 = cy[p], a1 = phi[p + sy], a2 = cy[p - sy], a3 = phi[p - sy], a4 = cx[p], a5 = phi[p + 1],
+            a6 = cx[p - 1], a7 = phi[p - 1], r = rhs[p], q = cc[p];
+    __builtin_amdgcn_sched_barrier(0);
+    T acc = a0 * a1 + a2 * a3;
+    acc = acc + a4 * a5;
+    acc = acc + a6 * a7;
+    acc = acc - r;
+    phi[p] = acc / q;
+}
+
+template <typename T>
+__device__ __forceinline__ void rb3_point_pre(T* __restrict__ phi, const T* __restrict__ rhs,
+                                              const T* __restrict__ cx, const T* __restrict__ cy,
+                                              const T* __restrict__ cz, const T* __restrict__ cc, long p,
+                                              long sy, long sz)
+{
+    const T a0 = cz[p], a1 = phi[p + sz], a2 = cz[p - sz], a3 = phi[p - sz], a4 = cy[p], a5 = phi[p + sy],
+            a6 = cy[p - sy], a7 = phi[p - sy], a8 = cx[p], a9 = phi[p + 1], a10 = cx[p - 1], a11 = phi[p - 1],
+            r = rhs[p], q = cc[p];
+    __builtin_amdgcn_sched_barrier(0);
+    T acc = a0 * a1 + a2 * a3;
+    acc = acc + a4 * a5;
+    acc = acc + a6 * a7;
+    acc = acc + a8 * a9;
+    acc = acc + a10 * a11;
+    acc = acc - r;
+    phi[p] = acc / q;
+}
+
+// restrict2_at without D_xy terms: the operands of resid2 at the five points
+template <typename T>
+__device__ __forceinline__ T restrict2_at_pre(const T* __restrict__ phi, const T* __restrict__ rhs,
+                                              const T* __restrict__ cx, const T* __restrict__ cy,
+                                              const T* __restrict__ cc, long c, long sy)
+{
+    struct Ops {
+        T v[11];
+    };
+    auto load = [&](long p) {
+        return Ops{{cc[p], phi[p], cx[p - 1], phi[p - 1], cx[p], phi[p + 1], cy[p - sy], phi[p - sy], cy[p],
+                    phi[p + sy], rhs[p]}};
+    };
+    auto res = [](const Ops& o) {
+        T r = o.v[0] * o.v[1];
+        r = r - o.v[2] * o.v[3];
+        r = r - o.v[4] * o.v[5];
+        r = r - o.v[6] * o.v[7];
+        r = r - o.v[8] * o.v[9];
+        return r + o.v[10];
+    };
+    const Ops a = load(c + sy + 1), b = load(c + sy - 1), d = load(c - sy + 1), e = load(c - sy - 1), f = load(c);
+    __builtin_amdgcn_sched_barrier(0);
+    T s = res(a) + res(b);
+    s = s + res(d);
+    s = s + res(e);
+    const T rcen = res(f);
+    return (T)0.0625 * s + (T)0.25 * rcen;
+}
+
 // Point i of an op's index space: LDS levels enumerate power-of-two extents
 // with shifts (points past the last row / plane are skipped by the caller),
 // global levels the exact extents with divisions.
@@ -1339,7 +1407,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
                     const int z = 2 + zr.hi, y = 2 + yx.hi;
                     const int x = 2 + ((par + y + z) & 1) + 2 * yx.lo;
                     if (x > g.mx || y > g.my || z > g.mz) continue;
-                    rb3_point(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, F.at(z, y, x), F.sy, F.sz);
+                    rb3_point_pre(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, F.at(z, y, x), F.sy, F.sz);
                 }
             } else {
                 const int cnt = xh * ny;
@@ -1348,7 +1416,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
                     const int y = 2 + yx.hi;
                     const int x = 2 + ((par + y) & 1) + 2 * yx.lo;
                     if (x > g.mx || y > g.my) continue;
-                    rb2_point(F.phi, F.rhs, F.cx, F.cy, F.cc, F.at(y, x), F.sy);
+                    rb2_point_pre(F.phi, F.rhs, F.cx, F.cy, F.cc, F.at(y, x), F.sy);
                 }
             }
         };
@@ -1392,8 +1460,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
                 if (K > gc.mx || J > gc.my) continue;
                 const long c = F.at(2 * J - 1, 2 * K - 1);
                 const T v = F.od ? restrict2_at(F.phi, F.rhs, F.cx, F.cy, F.cc, F.drr, F.dlr, c, F.sy)
-                                 : restrict2_at(F.phi, F.rhs, F.cx, F.cy, F.cc, (const T*)nullptr,
-                                                (const T*)nullptr, c, F.sy);
+                                 : restrict2_at_pre(F.phi, F.rhs, F.cx, F.cy, F.cc, c, F.sy);
                 C.rhs[C.at(J, K)] = v;
             }
         }
@@ -1461,8 +1528,11 @@ __device__ __forceinline__ void coarse_copy(D dst, S src, int n, int tid)
 // copied to LDS offset 0; levels
 // [lds_from, lds_to) are in LDS; wb_level (-1: none) is the start level when
 // it is staged.
+// One workgroup per CU: at most NT / 256 waves per SIMD, which lets the
+// scheduler spend registers on latency (loads issued early) instead of on an
+// occupancy the launch never reaches.
 template <typename T, int NT, int dim>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, NT / 256)))
 k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops, int nops, int ops_at,
               const int* __restrict__ act, int lds_from, int lds_to, int wb_level, const T* __restrict__ img,
               int img_n, long img_bs)
