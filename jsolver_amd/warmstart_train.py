@@ -9,7 +9,7 @@ seeded per batch) and solved by the project's own batched FP64 multigrid
 the training targets.
 
     python -m jsolver_amd.warmstart_train --samples 20000 --m 256 --epochs 4 \
-        --out gpurun_out/unet.pt --eval-m 512 --eval-batch 64
+        --out unet.pt --eval-m 512 --eval-batch 64
 """
 import argparse
 import json
