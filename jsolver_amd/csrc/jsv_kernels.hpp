@@ -1771,7 +1771,21 @@ k_sum_partials(const double* __restrict__ partials, long n, double* __restrict__
     __shared__ double sh[16];
     double s = 0.0;
     const double* seg = partials + (long)blockIdx.x * n;
-    for (long i = threadIdx.x; i < n; i += 1024) s += seg[i];
+    // each thread adds its elements in increasing index order; eight loads are
+    // issued before their adds, so the serial sum waits on one memory latency
+    // per eight elements instead of one per element
+    constexpr int U = 8;
+    for (long i0 = threadIdx.x; i0 < n; i0 += 1024L * U) {
+        double v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const long i = i0 + 1024L * k;
+            v[k] = i < n ? seg[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (i0 + 1024L * k < n) s += v[k];
+    }
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
