@@ -553,6 +553,9 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
                 hipLaunchKernelGGL((jsv::k_resid_first3<T, RB>), gt, dim3(jsv::BX, RB), 0, s, H.res[l],
                                    H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 0);
             }
+            // the gather indexes the compact residuals in 32 bits
+            if ((long)(gf.mz - 1) * (gf.my - 1) * (gf.mx / 2) >= (1L << 32))
+                return fail("level %d too large for the two-pass restriction", l);
             hipLaunchKernelGGL(jsv::k_restrict_gather3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
                                H.rhs[l + 1], H.res[l], gf, gc, zphi, rz);
         }

@@ -1084,9 +1084,14 @@ k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, 
     const int J = 2 + bk.y * BY + threadIdx.y;
     const int K = 2 + bk.x * BX + threadIdx.x;
     if (J > gc.my || K > gc.mx) return;
-    // residual at fine offset (dz, dy, dx) from the centre (2I-1, 2J-1, 2K-1)
+    // residual at fine offset (dz, dy, dx) from the centre (2I-1, 2J-1, 2K-1):
+    // res3_at with the row offsets formed once, in 32 bits (the compact array
+    // holds fewer than 2^32 elements; the host checks), and (x-2)/2 = K-1 for
+    // dx = +1, K-2 for dx = 0 and dx = -1
+    const unsigned mxh = (unsigned)(gf.mx >> 1), pz = (unsigned)(gf.my - 1) * mxh;
+    const unsigned c0 = ((unsigned)(2 * I - 3) * (unsigned)(gf.my - 1) + (unsigned)(2 * J - 3)) * mxh;
     auto R = [&](int dz, int dy, int dx) -> T {
-        return r[res3_at(gf, 2 * I - 1 + dz, 2 * J - 1 + dy, 2 * K - 1 + dx)];
+        return r[c0 + (unsigned)dz * pz + (unsigned)dy * mxh + (unsigned)(dx > 0 ? K - 1 : K - 2)];
     };
     T e = R(0, 1, 1) + R(0, 1, -1);
     e = e + R(0, -1, 1);
