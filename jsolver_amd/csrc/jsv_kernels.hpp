@@ -1507,25 +1507,37 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
     }
 }
 
-// n elements global -> LDS (or back), U loads in flight per thread
-template <typename T, int NT, typename D, typename S>
-__device__ __forceinline__ void coarse_copy(D dst, S src, int n, int tid)
-{
-    constexpr int U = 8;
-    for (int i0 = 0; i0 < n; i0 += NT * U) {
-        T v[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            const int i = i0 + k * NT + tid;
-            v[k] = src(i < n ? i : n - 1);
+// Coordinates (z, y, x) of linear index i over planes of ny rows of nx,
+// advanced by a fixed stride: one division at the start, none per step.
+struct Walk3 {
+    int z, y, x, dz, dy, dx, ny, nx;
+    __device__ __forceinline__ Walk3(int i, int stride, int ny_, int nx_) : ny(ny_), nx(nx_)
+    {
+        const int nxy = ny * nx;
+        z = i / nxy;
+        const int r = i - z * nxy;
+        y = r / nx;
+        x = r - y * nx;
+        dz = stride / nxy;
+        const int q = stride - dz * nxy;
+        dy = q / nx;
+        dx = q - dy * nx;
+    }
+    __device__ __forceinline__ void next()
+    {
+        x += dx;
+        y += dy;
+        z += dz;
+        if (x >= nx) {
+            x -= nx;
+            ++y;
         }
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            const int i = i0 + k * NT + tid;
-            if (i < n) dst(i, v[k]);
+        if (y >= ny) {
+            y -= ny;
+            ++z;
         }
     }
-}
+};
 
 // ops: nops packed op words, copied to LDS after the levels (element offset
 // ops_at of T); img: this launch's image of the staged levels (img_n
@@ -1567,6 +1579,8 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops,
         constexpr int UO = 2, UR = 8;
         auto cdiv = [](int a, int q) { return (a + q - 1) / q; };
         const int rounds = max(cdiv(nops, NT * UO), cdiv(nr, NT * UR));
+        const long rsz = W != nullptr ? W->g.sz : 0, rsy = W != nullptr ? W->g.sy : 0;
+        Walk3 wr(tid, NT, W != nullptr ? W->lsz / W->lsy : 1, W != nullptr ? W->lsy : 1);
         for (int r = 0; r < rounds; ++r) {
             unsigned vo[UO];
             T vr[UR];
@@ -1578,11 +1592,13 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops,
             if (nr > 0) {
 #pragma unroll
                 for (int k = 0; k < UR; ++k) {
-                    int i = (r * UR + k) * NT + tid;
-                    i = i < nr ? i : nr - 1;
-                    // compact point i is padded point (z, y, x) + 1
-                    const int z = i / W->lsz, q = i - z * W->lsz, y = q / W->lsy, x = q - y * W->lsy;
-                    vr[k] = rsrc[(dim == 3 ? (long)(z + 1) * W->g.sz : 0) + (long)(y + 1) * W->g.sy + x + 1];
+                    const int i = (r * UR + k) * NT + tid;
+                    // compact point i is padded point (z, y, x) + 1; past the
+                    // end the load reads point 0 (value unused)
+                    const bool in = i < nr;
+                    const int z = in ? wr.z : 0, y = in ? wr.y : 0, x = in ? wr.x : 0;
+                    vr[k] = rsrc[(dim == 3 ? (long)(z + 1) * rsz : 0) + (long)(y + 1) * rsy + x + 1];
+                    wr.next();
                 }
             }
 #pragma unroll
@@ -1636,15 +1652,25 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops,
         T* dst = W.phi + (dim == 2 ? (long)b * W.g.sz : 0);
         const T* s = sm + W.lds[0];
         const int nx = W.g.nx, ny = W.g.ny;
-        coarse_copy<T, NT>([&](int i, T v) { dst[i] = v; },
-                           [&](int i) {
-                               const int z = dim == 3 ? i / (nx * ny) : 1, r = dim == 3 ? i - (z * nx * ny) : i;
-                               const int y = r / nx, x = r - y * nx;
-                               const bool in = z >= 1 && y >= 1 && x >= 1 && y <= ny - 2 && x <= nx - 2 &&
-                                               (dim == 2 || z <= W.g.nz - 2);
-                               return in ? s[(dim == 3 ? (z - 1) * W.lsz : 0) + (y - 1) * W.lsy + x - 1] : (T)0;
-                           },
-                           np, tid);
+        constexpr int U = 8;
+        Walk3 w(tid, NT, ny, nx);
+        for (int i0 = 0; i0 < np; i0 += NT * U) {
+            T v[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                // padded point (z, y, x); 2D: one plane (w.z == 0 inside the array)
+                const int z = dim == 3 ? w.z : (w.z == 0 ? 1 : 0), y = w.y, x = w.x;
+                const bool in = z >= 1 && y >= 1 && x >= 1 && y <= ny - 2 && x <= nx - 2 &&
+                                (dim == 2 || z <= W.g.nz - 2);
+                v[k] = in ? s[(dim == 3 ? (z - 1) * W.lsz : 0) + (y - 1) * W.lsy + x - 1] : (T)0;
+                w.next();
+            }
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                const int i = i0 + k * NT + tid;
+                if (i < np) dst[i] = v[k];
+            }
+        }
     }
 }
 
