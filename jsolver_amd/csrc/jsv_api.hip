@@ -684,25 +684,24 @@ bool op_empty(const jsv_plan* P, const std::vector<Geo>& G, const jsv::CoarseOp&
     return true;
 }
 
-// Loop count of one op in k_coarse_prog, for the one-wave runs: exact
-// extents on global levels, power-of-two extents on staged ones (whose arrays
-// are compact, ln elements).
+// Loop count of one op in k_coarse_prog, for the one-wave runs: power-of-two
+// extents (the resolutions); a zeroing op covers the whole array (compact on
+// staged levels, ln elements).
 long op_points(const jsv_plan* P, const std::vector<Geo>& G, const std::vector<int>& ln, bool staged,
                const jsv::CoarseOp& op)
 {
     const Geo& g = G[op.level];
-    const int e = staged ? 0 : 1;
-    const long nz = P->dim == 3 ? g.mz - e : 1;
+    const long nz = P->dim == 3 ? g.mz : 1;
     const long full = staged ? ln[op.level] : (P->dim == 3 ? g.n : g.sz);
     switch (op.kind) {
     case jsv::CO_RB:
-    case jsv::CO_RB_PAIR: return (long)(g.mx / 2) * (g.my - e) * nz;
+    case jsv::CO_RB_PAIR: return (long)(g.mx / 2) * g.my * nz;
     case jsv::CO_RB_DIAG: return (long)(g.mx / 2) * (g.my / 2);
     case jsv::CO_ZERO: return full;
-    case jsv::CO_PROLONG: return (long)(g.mx - e) * (g.my - e) * nz;
+    case jsv::CO_PROLONG: return (long)g.mx * g.my * nz;
     default: {
         const Geo& gc = G[op.level + 1];
-        const long inner = (long)(gc.mx - e) * (gc.my - e) * (P->dim == 3 ? gc.mz - e : 1);
+        const long inner = (long)gc.mx * gc.my * (P->dim == 3 ? gc.mz : 1);
         const long zc = op.b ? (staged ? ln[op.level + 1] : (P->dim == 3 ? gc.n : gc.sz)) : 0;
         return inner > zc ? inner : zc;
     }

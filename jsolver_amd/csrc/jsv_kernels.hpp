@@ -1377,17 +1377,15 @@ __device__ __forceinline__ T restrict2_at_pre(const T* __restrict__ phi, const T
     return (T)0.0625 * s + (T)0.25 * rcen;
 }
 
-// Point i of an op's index space: LDS levels enumerate power-of-two extents
-// with shifts (points past the last row / plane are skipped by the caller),
-// global levels the exact extents with divisions.
+// Point i of an op's index space: every level enumerates power-of-two extents
+// (resolutions are powers of two) with shifts; points past the last row /
+// plane are skipped by the caller.
 struct Ix2 {
     int hi, lo;
 };
-template <bool POW2>
 __device__ __forceinline__ Ix2 split2(int i, int nlo, int llo)
 {
-    if constexpr (POW2) return {i >> llo, i & (nlo - 1)};
-    else return {i / nlo, i % nlo};
+    return {i >> llo, i & (nlo - 1)};
 }
 
 // One op over points t0, t0 + nt, ...; FL / CL: level / level + 1 in LDS.
@@ -1398,17 +1396,16 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
     const LvPtr<T>& LF = lv[op.level];
     const Geo g = LF.g;
     const LvView<T, FL> F(LF, sm, (dim == 2 && !FL) ? (long)b * g.sz : 0, dim);
-    // extents of the loops: exact on global levels, padded to powers of two on LDS ones
-    auto ext = [&](int m) { return FL ? m : m - 1; };
+    // loop extents are the resolutions: padded past the last unknown to a power of two
     auto lg = [](int v) { return v > 0 ? __builtin_ctz((unsigned)v) : 0; };
     if (op.kind == CO_RB || op.kind == CO_RB_PAIR) {
-        const int xh = g.mx / 2, ny = ext(g.my), lxh = lg(xh), lny = lg(ny);
+        const int xh = g.mx / 2, ny = g.my, lxh = lg(xh), lny = lg(ny);
         auto half = [&](int par) {
             if constexpr (dim == 3) {
-                const int cnt = xh * ny * ext(g.mz);
+                const int cnt = xh * ny * g.mz;
                 for (int i = t0; i < cnt; i += nt) {
-                    const Ix2 zr = split2<FL>(i, xh * ny, lxh + lny);
-                    const Ix2 yx = split2<FL>(zr.lo, xh, lxh);
+                    const Ix2 zr = split2(i, xh * ny, lxh + lny);
+                    const Ix2 yx = split2(zr.lo, xh, lxh);
                     const int z = 2 + zr.hi, y = 2 + yx.hi;
                     const int x = 2 + ((par + y + z) & 1) + 2 * yx.lo;
                     if (x > g.mx || y > g.my || z > g.mz) continue;
@@ -1417,7 +1414,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
             } else {
                 const int cnt = xh * ny;
                 for (int i = t0; i < cnt; i += nt) {
-                    const Ix2 yx = split2<FL>(i, xh, lxh);
+                    const Ix2 yx = split2(i, xh, lxh);
                     const int y = 2 + yx.hi;
                     const int x = 2 + ((par + y) & 1) + 2 * yx.lo;
                     if (x > g.mx || y > g.my) continue;
@@ -1436,7 +1433,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
     } else if (op.kind == CO_RB_DIAG) {
         const int xh = g.mx / 2, yh = g.my / 2, lxh = lg(xh);
         for (int i = t0; i < xh * yh; i += nt) {
-            const Ix2 yx = split2<FL>(i, xh, lxh);
+            const Ix2 yx = split2(i, xh, lxh);
             const int y = (op.b ? 2 : 3) + 2 * yx.hi;
             const int x = (op.a ? 2 : 3) + 2 * yx.lo;
             if (y > g.my || x > g.mx) continue;
@@ -1446,12 +1443,12 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
         const LvPtr<T>& LC = lv[op.level + 1];
         const Geo gc = LC.g;
         const LvView<T, CL> C(LC, sm, (dim == 2 && !CL) ? (long)b * gc.sz : 0, dim);
-        const int nx = ext(gc.mx), ny = ext(gc.my), lnx = lg(nx), lny = lg(ny);
+        const int nx = gc.mx, ny = gc.my, lnx = lg(nx), lny = lg(ny);
         if constexpr (dim == 3) {
-            const int cnt = nx * ny * ext(gc.mz);
+            const int cnt = nx * ny * gc.mz;
             for (int i = t0; i < cnt; i += nt) {
-                const Ix2 zr = split2<FL>(i, nx * ny, lnx + lny);
-                const Ix2 yx = split2<FL>(zr.lo, nx, lnx);
+                const Ix2 zr = split2(i, nx * ny, lnx + lny);
+                const Ix2 yx = split2(zr.lo, nx, lnx);
                 const int I = 2 + zr.hi, J = 2 + yx.hi, K = 2 + yx.lo;
                 if (K > gc.mx || J > gc.my || I > gc.mz) continue;
                 C.rhs[C.at(I, J, K)] =
@@ -1460,7 +1457,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
             }
         } else {
             for (int i = t0; i < nx * ny; i += nt) {
-                const Ix2 yx = split2<FL>(i, nx, lnx);
+                const Ix2 yx = split2(i, nx, lnx);
                 const int J = 2 + yx.hi, K = 2 + yx.lo;
                 if (K > gc.mx || J > gc.my) continue;
                 const long c = F.at(2 * J - 1, 2 * K - 1);
@@ -1477,12 +1474,12 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
         const LvPtr<T>& LC = lv[op.level + 1];
         const Geo gc = LC.g;
         const LvView<T, CL> C(LC, sm, (dim == 2 && !CL) ? (long)b * gc.sz : 0, dim);
-        const int nx = ext(g.mx), ny = ext(g.my), lnx = lg(nx), lny = lg(ny);
+        const int nx = g.mx, ny = g.my, lnx = lg(nx), lny = lg(ny);
         if constexpr (dim == 3) {
-            const int cnt = nx * ny * ext(g.mz);
+            const int cnt = nx * ny * g.mz;
             for (int i = t0; i < cnt; i += nt) {
-                const Ix2 zr = split2<FL>(i, nx * ny, lnx + lny);
-                const Ix2 yx = split2<FL>(zr.lo, nx, lnx);
+                const Ix2 zr = split2(i, nx * ny, lnx + lny);
+                const Ix2 yx = split2(zr.lo, nx, lnx);
                 const int z = 2 + zr.hi, y = 2 + yx.hi, x = 2 + yx.lo;
                 if (x > g.mx || y > g.my || z > g.mz) continue;
                 T corr;
@@ -1494,7 +1491,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
             }
         } else {
             for (int i = t0; i < nx * ny; i += nt) {
-                const Ix2 yx = split2<FL>(i, nx, lnx);
+                const Ix2 yx = split2(i, nx, lnx);
                 const int y = 2 + yx.hi, x = 2 + yx.lo;
                 if (x > g.mx || y > g.my) continue;
                 T corr;
