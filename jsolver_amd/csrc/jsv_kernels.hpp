@@ -1445,15 +1445,51 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
         const LvView<T, CL> C(LC, sm, (dim == 2 && !CL) ? (long)b * gc.sz : 0, dim);
         const int nx = gc.mx, ny = gc.my, lnx = lg(nx), lny = lg(ny);
         if constexpr (dim == 3) {
+            // Four lanes per coarse point (a wave holds whole groups): part p
+            // evaluates residuals 4p .. 4p+3 of restrict3_at's list, part 3 the
+            // centre; the group's first lane folds them in restrict3_at's order,
+            // so the value is restrict3_at's to the bit.
             const int cnt = nx * ny * gc.mz;
-            for (int i = t0; i < cnt; i += nt) {
+            const int part = t0 & 3, g0 = (t0 & (COARSE_WAVE - 1)) & ~3;
+            const long A = part == 0 ? F.sy : F.sz, B = part == 2 ? F.sy : 1;
+            for (int i4 = t0; i4 < 4 * cnt; i4 += nt) {
+                const int i = i4 >> 2;
                 const Ix2 zr = split2(i, nx * ny, lnx + lny);
                 const Ix2 yx = split2(zr.lo, nx, lnx);
                 const int I = 2 + zr.hi, J = 2 + yx.hi, K = 2 + yx.lo;
-                if (K > gc.mx || J > gc.my || I > gc.mz) continue;
-                C.rhs[C.at(I, J, K)] =
-                    restrict3_at(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, F.at(2 * I - 1, 2 * J - 1, 2 * K - 1), F.sy,
-                                 F.sz);
+                const bool ok = K <= gc.mx && J <= gc.my && I <= gc.mz;
+                T r0 = (T)0, r1 = (T)0, r2 = (T)0, r3 = (T)0;
+                if (ok) {
+                    const long c = F.at(2 * I - 1, 2 * J - 1, 2 * K - 1);
+                    auto R = [&](long o) { return resid3<T, T>(F.phi, F.rhs, F.cx, F.cy, F.cz, F.cc, c + o, F.sy, F.sz); };
+                    if (part < 3) {
+                        r0 = R(A + B);
+                        r1 = R(A - B);
+                        r2 = R(-A + B);
+                        r3 = R(-A - B);
+                    } else {
+                        r0 = R(0);
+                    }
+                }
+                const T a0 = __shfl(r0, g0 + 1), a1 = __shfl(r1, g0 + 1), a2 = __shfl(r2, g0 + 1),
+                        a3 = __shfl(r3, g0 + 1);
+                const T b0 = __shfl(r0, g0 + 2), b1 = __shfl(r1, g0 + 2), b2 = __shfl(r2, g0 + 2),
+                        b3 = __shfl(r3, g0 + 2);
+                const T rcen = __shfl(r0, g0 + 3);
+                if (ok && part == 0) {
+                    T e = r0 + r1;
+                    e = e + r2;
+                    e = e + r3;
+                    e = e + a0;
+                    e = e + a1;
+                    e = e + a2;
+                    e = e + a3;
+                    e = e + b0;
+                    e = e + b1;
+                    e = e + b2;
+                    e = e + b3;
+                    C.rhs[C.at(I, J, K)] = (T)0.03125 * e + (T)0.125 * rcen;
+                }
             }
         } else {
             for (int i = t0; i < nx * ny; i += nt) {
