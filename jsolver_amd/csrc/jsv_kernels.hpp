@@ -264,6 +264,28 @@ __device__ __forceinline__ bool prolong3_corr_at(Coarse C3, int x, int y, int z,
     return true;
 }
 
+// prolong3_corr_at at a point with x + y + z even, which has all three
+// coordinates even (eight terms) or exactly one (two terms along that axis):
+// the same terms in the same order, without the branches for the cases such a
+// point never takes.  Along an axis the terms sit at v/2 + 1 and v/2 + (v & 1),
+// which for an odd v are both (v + 1)/2.
+template <typename T, typename Coarse>
+__device__ __forceinline__ T prolong3_corr_even(Coarse C3, int x, int y, int z)
+{
+    const int Xl = x >> 1, Yl = y >> 1, Zl = z >> 1;
+    if ((x | y | z) & 1) {
+        return (T)0.5 * (C3(Zl + 1, Yl + 1, Xl + 1) + C3(Zl + (z & 1), Yl + (y & 1), Xl + (x & 1)));
+    }
+    T s = C3(Zl + 1, Yl + 1, Xl + 1) + C3(Zl, Yl + 1, Xl + 1);
+    s = s + C3(Zl + 1, Yl, Xl + 1);
+    s = s + C3(Zl, Yl, Xl + 1);
+    s = s + C3(Zl + 1, Yl + 1, Xl);
+    s = s + C3(Zl, Yl + 1, Xl);
+    s = s + C3(Zl + 1, Yl, Xl);
+    s = s + C3(Zl, Yl, Xl);
+    return (T)0.125 * s;
+}
+
 template <typename T>
 __device__ __forceinline__ bool prolong3_corr(const T* __restrict__ pc, long csy, long csz, int x,
                                               int y, int z, int full, T& corr)
@@ -293,12 +315,15 @@ k_rb3_pro(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ 
     if (x > g.mx) return;
     const long sy = g.sy, sz = g.sz;
     const long p = (long)z * sz + (long)y * sy + x;
+    // the neighbours have x + y + z = par + 1 (mod 2): even for par == 1
+    auto C3 = [&](int I, int J, int K) -> T { return pc[(long)I * gc.sz + (long)J * gc.sy + K]; };
     auto nb = [&](int xx, int yy, int zz, long q) -> T {
         T v = phi[q];
-        T corr;
-        if (xx >= 2 && xx <= g.mx && yy >= 2 && yy <= g.my && zz >= 2 && zz <= g.mz &&
-            prolong3_corr(pc, gc.sy, gc.sz, xx, yy, zz, 0, corr))
-            v = v + corr;
+        if (xx >= 2 && xx <= g.mx && yy >= 2 && yy <= g.my && zz >= 2 && zz <= g.mz) {
+            T corr;
+            if (par == 1) v = v + prolong3_corr_even<T>(C3, xx, yy, zz);
+            else if (prolong3_corr_at<T>(C3, xx, yy, zz, 0, corr)) v = v + corr;
+        }
         return v;
     };
     T acc = cz[p] * nb(x, y, z + 1, p + sz) + cz[p - sz] * nb(x, y, z - 1, p - sz);
@@ -615,8 +640,10 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         const int rr_ = i / nh, kk = i - rr_ * nh;
         const int yy = y0 - 2 + rr_;
         const int x = 2 * kk + ((yy + z) & 1);
-        return x >= 2 && x <= g.mx && yy >= 2 && yy <= g.my && z >= 2 && z <= g.mz &&
-               prolong3_corr_at<T>(cval, x, yy, z, 0, corr);
+        // a black point: x + yy + z is even
+        if (x < 2 || x > g.mx || yy < 2 || yy > g.my || z < 2 || z > g.mz) return false;
+        corr = prolong3_corr_even<T>(cval, x, yy, z);
+        return true;
     };
     auto corrected = [&](int z, int i, T v) -> T {
         if constexpr (PRO) {
