@@ -118,6 +118,29 @@ def _traffic():
         return None
 
 
+def _host_cpus():
+    """Threads for the CPU baseline: every CPU this process may run on
+    (sched affinity), capped by OMP_NUM_THREADS where the host sets it; plus
+    the host's CPU count and model for the record."""
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    except ValueError:
+        cap = 0
+    threads = max(1, min(share, cap) if cap > 0 else share)
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"cpu_count": os.cpu_count(), "affinity": share, "omp_num_threads": cap or None,
+                     "cpu_model": model}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -190,7 +213,7 @@ def main():
         try:
             sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
             from oracle import gmg, gmg_c
-            threads = max(1, min(16, os.cpu_count() or 1))
+            threads, host = _host_cpus()
             t0 = time.perf_counter()
             op = gmg.build_operator(3, (m, m, m), tuple(g.idel for g in grids), lam, [dxx, dyy, dzz])
             t_setup = time.perf_counter() - t0
@@ -204,10 +227,11 @@ def main():
                 done += 1
             t_cyc = t_run / done
             cpu = {"value": round(t_setup + t_cyc * n64, 2), "unit": "s", "cores": threads, "kind": "port",
+                   "extrapolated": True, "host": host,
                    "sample": f"the oracle's C restatement (FP64, OpenMP, bit-identical to the NumPy oracle): "
                              f"NumPy operator setup ({t_setup:.1f}s) + {done} V-cycles at {m}^3 timed "
-                             f"({t_cyc:.3f}s/cycle on {threads} threads), extrapolated to the FP64 GPU "
-                             f"cycle count {n64}"}
+                             f"({t_cyc:.3f}s/cycle on {threads} threads: every CPU this process may use), "
+                             f"extrapolated to the FP64 GPU cycle count {n64}"}
             log(f"cpu oracle (C, {threads} threads): {t_cyc:.3f}s/cycle")
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "s", "cores": 1, "kind": "port", "sample": f"failed: {e}"}

@@ -50,7 +50,9 @@ typedef struct jsv_config {
     int stop;               /* JSV_STOP_* (dynamic mode only) */
     double rtol;            /* relative-residual threshold for JSV_STOP_REL_RESIDUAL */
     int inner_cycles;       /* JSV_MIXED: FP32 cycles per FP64 refinement step;
-                               0 = predicted from the measured contraction (8..64) */
+                               0 = predicted from the measured contraction (8 first,
+                               then 1..64: to the tolerance, or in fixed mode to five
+                               decades below the current residual) */
     int check_every;        /* JSV_FP64 with JSV_STOP_REL_RESIDUAL: cycles between
                                residual checks (1 = every cycle, the exact count);
                                0 = spaced by the predicted cycles-to-tolerance */

@@ -5,7 +5,7 @@ along x (direction 0), y (1), z (2) or all (-1) set to zero or to the given
 values: vals_xb / vals_xe shaped ``(nz, ny)``, vals_yb / vals_ye shaped
 ``(nz, nx)``.  Faithful to the reference, the z faces also take vals_yb /
 vals_ye (read as ``(ny, nx)``) and vals_zb / vals_ze are accepted but unused
-(boundary_handler_3d.py:61-63).  The solver never passes values, so this only
+(boundary_handler_3d.py:31-33).  The solver never passes values, so this only
 concerns direct callers.  Runs the HIP kernel jsv_do_bcs_3d.
 """
 from . import _lib

@@ -168,6 +168,22 @@ int dalloc(jsv_plan* P, void** p, size_t bytes)
     return 0;
 }
 
+// frees a buffer from dalloc (nullptr: nothing); the caller has synchronised
+int dfree(jsv_plan* P, void* p, size_t bytes)
+{
+    if (p == nullptr) return 0;
+    for (size_t i = 0; i < P->allocs.size(); ++i)
+        if (P->allocs[i] == p) {
+            P->allocs.erase(P->allocs.begin() + (long)i);
+            P->bytes -= (int64_t)bytes;
+            HIPCHK(hipFree(p));
+            return 0;
+        }
+    return fail("internal: freeing a buffer the plan does not own");
+}
+
+void drop_graphs(jsv_plan* P);
+
 template <typename T>
 int alloc_hier(jsv_plan* P, Hier<T>& H, bool with_coeffs)
 {
@@ -804,7 +820,11 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         if (P->img_key[pi] != ikey) {
             const size_t nb = sizeof(T) * (size_t)img_n * (size_t)P->batch;
             if (P->img_cap[pi] < nb) {
+                // captured cycles hold the old image's address: drop them with it
                 HIPCHK(hipStreamSynchronize(s));
+                drop_graphs(P);
+                RETCHK(dfree(P, P->img_dev[pi], P->img_cap[pi]));
+                P->img_dev[pi] = nullptr;
                 RETCHK(dalloc(P, &P->img_dev[pi], nb));
                 P->img_cap[pi] = nb;
             }
@@ -904,6 +924,16 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
     return 0;
 }
 
+// The staged coefficient image holds the plan's current operator (or nothing
+// is staged).  A program may only run while this holds.
+template <typename T>
+bool image_current(const jsv_plan* P)
+{
+    const int pi = sizeof(T) == 4 ? 1 : 0;
+    if (P->img_n[pi] <= 0) return true;
+    return !P->img_key[pi].empty() && P->img_key[pi].back() == (const void*)P->op_version;
+}
+
 // multigrid_cycle (solver_2d.py:582-614 / solver_3d.py:588-620)
 // red_done: the restriction into this level already ran the first half of
 // its first sweep (fuse_halves)
@@ -920,6 +950,7 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
     const int start = coarse_start(P, H);
     const jsv_plan::CoarseProg* pr = (start == l + 1) ? find_prog<T>(P, c, fcyc, start) : nullptr;
     if (pr != nullptr && pr->ops == nullptr) pr = nullptr;
+    if (pr != nullptr && !image_current<T>(P)) pr = nullptr;   // staged coefficients of another operator
     // the first visit of level l + 1 starts from phi = 0 with a sweep: its first
     // half can run inside the restriction
     const int first_sweeps = (l + 1 == P->levels - 2) ? c.nu1 + c.nu2 : c.nu1;
@@ -1926,7 +1957,10 @@ static int solve_mixed(jsv_plan* P, const double* phi_in, const double* rhs, dou
             kb[b] = 0;
             if (!P->h_act[b]) continue;
             int64_t k = c->inner_cycles;
-            if (adaptive) k = relres ? predict_cycles(rel_prev[b], T.rel[b], k_last[b], thr, 8, 64) : 8;
+            // fixed mode has no tolerance: a step aims at the five decades an
+            // FP32 correction resolves
+            const double tgt = c->fixed ? T.rel[b] * 1e-5 : thr;
+            if (adaptive) k = relres ? predict_cycles(rel_prev[b], T.rel[b], k_last[b], tgt, 8, 64) : 8;
             if (T.count[b] + k > c->max_iters) k = c->max_iters - T.count[b];
             kb[b] = k;
             todo = k > todo ? k : todo;
@@ -2287,6 +2321,9 @@ int jsv_forward_checkpoints(jsv_plan* P, const double* phi0, const double* rhs, 
     if (c->skip_pre)
         for (int i = 0; i < c->nu1; ++i) RETCHK(smooth(P, H, 0, s));
     HIPCHK(hipMemcpyAsync(ckpt, H.phi[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    // the coarse program stages this operator's coarse coefficients (a program
+    // built by an earlier solve holds the operator of that solve)
+    RETCHK(prepare_coarse(P, H, *c, s));
     for (int64_t k = 0; k < c->max_iters; ++k) {
         RETCHK(zero_rim_level0(P, H.phi[0], s));
         RETCHK(mg_cycle(P, H, *c, 0, c->fcycle != 0, c->skip_pre != 0, s));

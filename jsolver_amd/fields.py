@@ -30,7 +30,7 @@ def grf(shape, h, corr_len, sigma, rng):
         sh[ax] = k.size
         k2 = k2 + (k.reshape(sh) ** 2)
     f *= np.exp(-2.0 * np.pi ** 2 * corr_len ** 2 * k2)
-    g = np.fft.irfftn(f, s=shape)
+    g = np.fft.irfftn(f, s=shape, axes=tuple(range(len(shape))))
     g -= g.mean()
     sd = g.std()
     return g * (sigma / sd if sd > 0 else 0.0)

@@ -110,8 +110,8 @@ def bcs_2d(arr, rim=1, direction=-1, vals_xb=None, vals_xe=None, vals_yb=None, v
 
 def bcs_3d(arr, rim=1, direction=-1, vals_xb=None, vals_xe=None, vals_yb=None,
            vals_ye=None, vals_zb=None, vals_ze=None):
-    """boundary_handler_3d.py:51-65.  Faithful to the reference, the z faces
-    take ``vals_yb``/``vals_ye`` (boundary_handler_3d.py:62-63); ``vals_z*``
+    """boundary_handler_3d.py:21-35.  Faithful to the reference, the z faces
+    take ``vals_yb``/``vals_ye`` (boundary_handler_3d.py:31-33); ``vals_z*``
     are accepted and unused."""
     a = np.array(arr, dtype=np.float64, copy=True)
     w = rim + 1

@@ -6,6 +6,11 @@
 * 2D: constant D arrays reproduce the scalar path; brute force on 8^2 and
   16x8 with and without D_xy.
 * Discrete sine eigenvectors are reproduced exactly (closed form).
+* do_BCs: the reference's own 2D tests (tests/test_boundary_handler_2d.py),
+  and a point-by-point statement of the 3D faces (direction, rim, values,
+  the z faces reading vals_y*).
+* Prolongation: affine fields are reproduced exactly wherever the
+  reference interpolates (3D bilinear faces only when nu2 <= 0).
 * The operator is symmetric without D_xy; ghost values of D, lambda and rhs
   have no effect.
 Each check is chosen so a plausible slip (a dropped face, a shifted index, a
@@ -316,3 +321,187 @@ def test_dirichlet_lift_matches_direct_solve(dim, ms, dxy, spatial):
         bad[2:-2, -3] = rl[2:-2, -3] + 0.5 * (rl[2:-2, -3] - rhs[2:-2, -3])
         pb, _ = gmg.solve(op, gmg.Config(fixed=True, max_iters_outer=300), np.zeros(shape), bad)
         assert np.max(np.abs(pb[tuple(mesh)] - x)) > 1e-3 * np.max(np.abs(x))
+
+
+# ---------------------------------------------------------------------------
+# do_BCs: the reference's own tests (tests/test_boundary_handler_2d.py:13-64)
+# restated against the oracle, and hand-built 3D expectations
+# ---------------------------------------------------------------------------
+def _pad(core, n_edge, dim):
+    """jnp.pad(ones(dim - 2 n_edge), (n_edge, n_edge)) as in the reference's tests."""
+    return np.pad(np.full(dim - 2 * n_edge, core), (n_edge, n_edge))
+
+
+def test_bcs_2d_reference_default():
+    dim = 131
+    new = gmg.bcs_2d(np.ones((dim, dim)))
+    for row in (new[0], new[1], new[-1], new[-2], new[:, 0], new[:, 1], new[:, -1], new[:, -2]):
+        assert np.all(row == np.zeros(dim))
+    # and nothing else changed
+    assert np.all(new[2:-2, 2:-2] == 1.0)
+
+
+def test_bcs_2d_reference_vals():
+    dim = 131
+    twos = np.ones(dim) * 2
+    new = gmg.bcs_2d(np.ones((dim, dim)), vals_xb=twos, vals_xe=twos, vals_yb=twos, vals_ye=twos)
+    for row in (new[0], new[1], new[-1], new[-2], new[:, 0], new[:, 1], new[:, -1], new[:, -2]):
+        assert np.all(row == twos)
+    assert np.all(new[2:-2, 2:-2] == 1.0)
+
+
+def test_bcs_2d_reference_xdir():
+    dim = 131
+    new = gmg.bcs_2d(np.ones((dim, dim)), direction=0)
+    for row in (new[0], new[1], new[-1], new[-2]):
+        assert np.all(row == _pad(1.0, 2, dim))
+    for col in (new[:, 0], new[:, 1], new[:, -1], new[:, -2]):
+        assert np.all(col == np.zeros(dim))
+
+
+def test_bcs_2d_reference_ydir():
+    dim = 131
+    new = gmg.bcs_2d(np.ones((dim, dim)), direction=1)
+    for row in (new[0], new[1], new[-1], new[-2]):
+        assert np.all(row == np.zeros(dim))
+    for col in (new[:, 0], new[:, 1], new[:, -1], new[:, -2]):
+        assert np.all(col == _pad(1.0, 2, dim))
+
+
+def _bcs_3d_by_points(arr, rim, direction, vxb, vxe, vyb, vye):
+    """Point-by-point statement of boundary_handler_3d.py:21-35: the x faces
+    are set first, then the y faces, then the z faces, each over the whole
+    face (later faces win on shared edges).  x faces take vals_x*[z, y],
+    y faces vals_y*[z, x], and the z faces also vals_y*, read as [y, x]
+    (boundary_handler_3d.py:31-33)."""
+    nz, ny, nx = arr.shape
+    w = rim + 1
+    out = arr.copy()
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                v = arr[z, y, x]
+                if direction in (-1, 0):
+                    if x < w:
+                        v = 0.0 if vxb is None else vxb[z, y]
+                    elif x >= nx - w:
+                        v = 0.0 if vxe is None else vxe[z, y]
+                if direction in (-1, 1):
+                    if y < w:
+                        v = 0.0 if vyb is None else vyb[z, x]
+                    elif y >= ny - w:
+                        v = 0.0 if vye is None else vye[z, x]
+                if direction in (-1, 2):
+                    if z < w:
+                        v = 0.0 if vyb is None else vyb[y, x]
+                    elif z >= nz - w:
+                        v = 0.0 if vye is None else vye[y, x]
+                out[z, y, x] = v
+    return out
+
+
+@pytest.mark.parametrize("rim", [1, 2])
+@pytest.mark.parametrize("direction", [-1, 0, 1, 2])
+@pytest.mark.parametrize("with_vals", [False, True])
+def test_bcs_3d_hand_built(rim, direction, with_vals):
+    rng = np.random.default_rng(40 + rim)
+    n = 9                       # nz == ny: the z faces read vals_y* as (ny, nx)
+    arr = rng.standard_normal((n, n, 11))
+    vals = [None] * 4
+    if with_vals:
+        # distinct values per face, so the order of the faces and the axes
+        # each face reads show up
+        vals = [rng.standard_normal((n, n)) + 10 * k for k in range(2)] + \
+               [rng.standard_normal((n, 11)) + 10 * k for k in range(2, 4)]
+    got = gmg.bcs_3d(arr, rim=rim, direction=direction, vals_xb=vals[0], vals_xe=vals[1],
+                     vals_yb=vals[2], vals_ye=vals[3],
+                     vals_zb=None if not with_vals else np.full((n, 11), -99.0),
+                     vals_ze=None if not with_vals else np.full((n, 11), -99.0))
+    want = _bcs_3d_by_points(arr, rim, direction, *vals)
+    np.testing.assert_array_equal(got, want)
+    if with_vals and direction in (-1, 2):
+        # the z faces hold vals_yb / vals_ye, never vals_zb / vals_ze
+        assert not np.any(got == -99.0)
+        np.testing.assert_array_equal(got[0], vals[2])
+        np.testing.assert_array_equal(got[-1], vals[3])
+
+
+def test_bcs_3d_pointwise_statement_has_power():
+    """The point-by-point statement separates the plausible misreadings: z
+    faces from vals_z*, and x faces set last."""
+    rng = np.random.default_rng(7)
+    n = 7
+    arr = rng.standard_normal((n, n, n))
+    v = [rng.standard_normal((n, n)) + 10 * k for k in range(4)]
+    want = _bcs_3d_by_points(arr, 1, -1, *v)
+    wrong_z = gmg.bcs_3d(arr, vals_xb=v[0], vals_xe=v[1], vals_yb=v[2], vals_ye=v[3])
+    wrong_z[:2] = v[0][None]          # z faces from another array
+    assert not np.array_equal(wrong_z, want)
+    x_last = gmg.bcs_3d(arr, vals_xb=v[0], vals_xe=v[1], vals_yb=v[2], vals_ye=v[3])
+    x_last[:, :, :2] = v[0][:, :, None]
+    x_last[:, :, -2:] = v[1][:, :, None]
+    assert not np.array_equal(x_last, want)
+
+
+# ---------------------------------------------------------------------------
+# prolongation: linear interpolation reproduces affine fields exactly
+# (solver_2d.py:649-668, solver_3d.py:654-677)
+# ---------------------------------------------------------------------------
+def _affine(shape, coef):
+    """coef[0] + sum_a coef[a+1] * (index along array axis a), dyadic so every
+    interpolation is exact in FP64."""
+    idx = np.meshgrid(*[np.arange(n, dtype=float) for n in shape], indexing="ij")
+    return coef[0] + sum(c * i for c, i in zip(coef[1:], idx))
+
+
+def _coarse_from_fine_affine(op, level, coef):
+    """Coarse level values whose fine point 2I-1 sits at coarse index I: the
+    affine function of the fine index, sampled at the coarse points."""
+    fine_shape = op.lv[level].shape
+    coarse_shape = op.lv[level + 1].shape
+    pc = np.zeros(coarse_shape)
+    inner = tuple(slice(1, n - 1) for n in coarse_shape)
+    # coarse index I (1..mc+1) <-> fine index 2I - 1
+    idx = np.meshgrid(*[2.0 * np.arange(1, n - 1) - 1.0 for n in coarse_shape], indexing="ij")
+    pc[inner] = coef[0] + sum(c * i for c, i in zip(coef[1:], idx))
+    return pc, _affine(fine_shape, coef)
+
+
+@pytest.mark.parametrize("dim,ms", [(2, (16, 8)), (3, (8, 8, 16))])
+@pytest.mark.parametrize("nu2", [0, -1, 1])
+def test_prolongation_reproduces_affine_fields(dim, ms, nu2):
+    lam = 0.0
+    D = [1.0] * dim
+    op = gmg.build_operator(dim, ms, _idels(ms), lam, D)
+    coef = [0.5, 0.25, -0.125, 0.375][:dim + 1]
+    for level in range(op.levels - 2):
+        pc, want = _coarse_from_fine_affine(op, level, coef)
+        c = gmg.prolong(op, level, pc, nu2)
+        inner = (slice(1, -1),) * dim
+        got, ref = c[inner], want[inner]
+        # points with exactly two even coordinates (3D) / both even (2D) are
+        # interpolated only when nu2 <= 0; otherwise they stay 0 (the red
+        # half of the post-smoothing sweep overwrites them)
+        ev = np.meshgrid(*[(np.arange(n) % 2 == 0) for n in c.shape], indexing="ij")
+        neven = sum(e.astype(int) for e in ev)[inner]
+        skipped = (neven == 2) if nu2 > 0 else np.zeros_like(neven, dtype=bool)
+        np.testing.assert_array_equal(got[~skipped], ref[~skipped])
+        assert np.all(got[skipped] == 0.0)
+        # ghost layers untouched
+        assert np.all(c[(0,) + (slice(None),) * (dim - 1)] == 0.0)
+
+
+def test_prolongation_affine_pin_has_power():
+    """A variant with one weight of the 3D bilinear faces changed fails the
+    affine check."""
+    ms = (8, 8, 8)
+    op = gmg.build_operator(3, ms, _idels(ms), 0.0, [1.0] * 3)
+    coef = [0.5, 0.25, -0.125, 0.375]
+    pc, want = _coarse_from_fine_affine(op, 0, coef)
+    c = gmg.prolong(op, 0, pc, 0)
+    O, E, Pp, Pm = slice(1, -1, 2), slice(2, -1, 2), slice(3, None, 2), slice(1, -2, 2)
+    bad = c.copy()
+    bad[E, O, E] = 0.25 * (c[Pp, O, Pp] + c[Pm, O, Pp] + c[Pp, O, Pm] + c[Pp, O, Pm])
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(c[inner], want[inner])
+    assert not np.array_equal(bad[inner], want[inner])
