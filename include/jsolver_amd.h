@@ -213,6 +213,19 @@ int jsv_residual(jsv_plan *plan, const double *phi, const double *rhs, double *o
 /* max |a - b| over n elements (distance, solver_2d.py:481-494); host result */
 int jsv_max_abs_diff(const double *a, const double *b, int64_t n, double *out, void *stream);
 
+/* jsv_max_abs_diff that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc).  out[s] (host, nseg doubles) receives
+ * max |a - b| over segment s of n elements (elements s*n .. s*n + n - 1);
+ * nseg = 1 gives jsv_max_abs_diff's result.  The result words live in a
+ * fixed device array of JSV_NOALLOC_MAX_SEGMENTS words per GPU, held until
+ * the process ends; more segments are rejected with an error.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other host
+ * threads, other streams) race.  Synchronises the stream. */
+#define JSV_NOALLOC_MAX_SEGMENTS 1024
+int jsv_max_abs_diff_noalloc(const double *a, const double *b, int64_t n, int64_t nseg, double *out,
+                             void *stream);
+
 /* ---- boundary handler (do_BCs, boundary_handler_2d.py:3-29 /
  * boundary_handler_3d.py:3-35), in place on a caller array.  vals_* may be
  * NULL (zero).  3D: like the reference, the z faces take vals_yb/vals_ye. ---- */

@@ -31,6 +31,7 @@ JSV_TUNE_COARSE_THREADS = 13
 JSV_TUNE_FUSE_HALVES = 14
 JSV_BUF_PHI = 0
 JSV_BUF_RHS = 1
+JSV_NOALLOC_MAX_SEGMENTS = 1024
 
 
 class JsvConfig(ctypes.Structure):
@@ -91,6 +92,7 @@ SIGNATURES = {
     "jsv_cycle": (_I, [_P, _I, ctypes.POINTER(JsvConfig), _I, _P]),
     "jsv_residual": (_I, [_P, _P, _P, _P, _DP, _P]),
     "jsv_max_abs_diff": (_I, [_P, _P, _I64, _DP, _P]),
+    "jsv_max_abs_diff_noalloc": (_I, [_P, _P, _I64, _I64, _DP, _P]),
     "jsv_dirichlet_lift": (_I, [_P, _P, _P, _P, _P]),
     "jsv_do_bcs_2d": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "jsv_do_bcs_3d": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -1656,6 +1657,67 @@ int jsv_max_abs_diff(const double* a, const double* b, int64_t n, double* out, v
     double v;
     std::memcpy(&v, &h, sizeof(v));
     *out = v;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// jsv_max_abs_diff without allocation: its result words live in a
+// file-scope device array (each GPU holds its own copy until the process
+// ends); the host reaches it through the address hipGetSymbolAddress returns
+// on that device, looked up once per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned long long scratch[JSV_NOALLOC_MAX_SEGMENTS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+// this device's address of `scratch` (the current device)
+int scratch_ptr(unsigned long long** out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail("device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) return fail("hipGetSymbolAddress returned no address");
+        g_scratch_ptr[dev] = (unsigned long long*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int jsv_max_abs_diff_noalloc(const double* a, const double* b, int64_t n, int64_t nseg, double* out,
+                             void* stream)
+{
+    if (a == nullptr || b == nullptr || out == nullptr) return fail("NULL argument");
+    if (n < 0) return fail("n must be >= 0 (got %lld)", (long long)n);
+    if (nseg < 1 || nseg > JSV_NOALLOC_MAX_SEGMENTS)
+        return fail("jsv_max_abs_diff_noalloc: %lld segments requested, the scratch holds at most %d",
+                    (long long)nseg, JSV_NOALLOC_MAX_SEGMENTS);
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* d = nullptr;
+    RETCHK(scratch_ptr(&d));
+    // the words keep the last call's maxima: start from 0 (a fill kernel, no memset)
+    hipLaunchKernelGGL(jsv::k_fill<unsigned long long>, dim3(1), dim3(256), 0, s, d, (long)nseg, 0ull);
+    long nb = (long)((n + 255) / 256);
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(jsv::k_maxabsdiff<double>, dim3((unsigned)nb, (unsigned)nseg), dim3(256), 0, s, a, b,
+                       (long)n, d);
+    LAUNCHCHK();
+    // non-negative doubles travel as their bit patterns
+    static_assert(sizeof(double) == sizeof(unsigned long long), "64-bit words");
+    HIPCHK(hipMemcpyAsync(out, d, sizeof(double) * (size_t)nseg, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
 
