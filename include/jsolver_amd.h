@@ -247,7 +247,7 @@ int jsv_do_bcs_3d(double *arr, int nz, int ny, int nx, int rim, int direction,
 int jsv_dirichlet_lift(jsv_plan *plan, const double *phi_b, const double *rhs, double *rhs_out,
                        void *stream);
 
-/* ---- reverse pass of fixed-mode solves (FP64, 5/7-point operators) ----
+/* ---- reverse pass of fixed-mode solves (FP64; 2D with or without D_xy, 3D) ----
  * Replaces the reference's jax.vjp over a fixed solve
  * (examples/benchmark.py:1184-1226; tests/test_solver_2d.py:200-207).
  * jsv_forward_checkpoints runs cfg->max_iters cycles from phi0 and stores the
@@ -255,14 +255,26 @@ int jsv_dirichlet_lift(jsv_plan *plan, const double *phi_b, const double *rhs, d
  * (max_iters + 1) x n doubles (n = fine array size).  jsv_adjoint takes the
  * cotangent g_out of the final phi and returns the cotangents of phi0, rhs,
  * lambda and the D components (any output may be NULL); each cycle's interior
- * states are recomputed from ckpt, not stored.  The scalar-D / scalar-lambda
- * cases return the per-cell cotangent of the broadcast array (sum it for the
- * scalar).  D_xy is not supported. */
+ * states are recomputed from ckpt, not stored.  g_lam needs an array lambda
+ * and g_dxx.. array D on the plan (scalar parts: jsv_adjoint_full). */
 int jsv_forward_checkpoints(jsv_plan *plan, const double *phi0, const double *rhs,
                             const jsv_config *cfg, double *ckpt, void *stream);
 int jsv_adjoint(jsv_plan *plan, const double *phi0, const double *rhs, const double *ckpt,
                 const jsv_config *cfg, const double *g_out, double *g_phi0, double *g_rhs,
                 double *g_lam, double *g_dxx, double *g_dyy, double *g_dzz, void *stream);
+/* jsv_adjoint with every operator input of the reference's jax.vjp
+ * (tests/test_solver_2d.py:200-207 differentiates scalar lambda and D;
+ * examples/benchmark.py:1223-1229 also D_xy):
+ *   g_dxy     cotangent of the 2D D_xy array (plans whose operator has D_xy;
+ *             solver_2d.py:411-416, 686), or NULL;
+ *   g_scalar  device double[4], or NULL: the cotangents of the operator's
+ *             scalar parts, [0] lambda, [1..3] D_xx, D_yy, D_zz (solver_2d.py:
+ *             417-421); entries of parts set from arrays are 0.
+ * Array outputs of scalar parts are an error. */
+int jsv_adjoint_full(jsv_plan *plan, const double *phi0, const double *rhs, const double *ckpt,
+                     const jsv_config *cfg, const double *g_out, double *g_phi0, double *g_rhs,
+                     double *g_lam, double *g_dxx, double *g_dyy, double *g_dzz, double *g_dxy,
+                     double *g_scalar, void *stream);
 /* device bytes held by the plan's reverse-pass buffers (tape included) */
 int jsv_adjoint_device_bytes(const jsv_plan *plan, int64_t *bytes);
 

@@ -98,6 +98,8 @@ SIGNATURES = {
     "jsv_do_bcs_3d": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "jsv_forward_checkpoints": (_I, [_P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P]),
     "jsv_adjoint": (_I, [_P, _P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P, _P, _P, _P, _P, _P, _P]),
+    "jsv_adjoint_full": (_I, [_P, _P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                              _P]),
     "jsv_adjoint_device_bytes": (_I, [_P, ctypes.POINTER(_I64)]),
     "jsv_tangent": (_I, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(JsvConfig), _P, _P, _P]),
 }

@@ -284,34 +284,28 @@ def with_boundary(out, phi_b, dim, batched=False):
 def _solve_differentiable(dim, plan, dev, full_shape, phi, rhs, lam, D, D_xy, fixed, precision,
                           cfg_kw, return_history, return_info, levels):
     """Fixed-mode solve recorded for torch.autograd (backward = the hand-written
-    adjoint).  Gradients flow to tensor arguments: phi, rhs, lam (array or
-    0-d), D arrays."""
+    adjoint).  Gradients flow to tensor arguments: phi, rhs, lam and D (arrays
+    or 0-d scalars) and D_xy."""
     from . import adjoint
     if not fixed:
         raise ValueError("Reverse-mode differentiation needs fixed=True (a fixed number of cycles); "
                          "the dynamic stop rule is not differentiable.")
-    if D_xy is not None:
-        raise NotImplementedError("reverse-mode differentiation with D_xy is not supported")
     if precision != "fp64":
         raise ValueError("reverse-mode differentiation runs in precision='fp64'")
     if return_history:
         raise ValueError("return_history is not available for differentiated solves")
 
     def dev_t(v):
+        """float64 device tensor; 0-d for scalars; the autograd graph is kept."""
         if torch.is_tensor(v):
             return v.to(device=dev, dtype=torch.float64).contiguous()
         return torch.from_numpy(np.ascontiguousarray(np.asarray(v, dtype=np.float64))).to(dev)
 
-    if any(is_scalar(d) and torch.is_tensor(d) and d.requires_grad for d in D):
-        raise NotImplementedError("gradients with respect to scalar D are not supported; pass D arrays")
     phi_d, rhs_d = dev_t(phi), dev_t(rhs)
-    if is_scalar(lam):
-        lam_v = dev_t(lam).expand(full_shape).contiguous() if (torch.is_tensor(lam) and lam.requires_grad) \
-            else scalar_value(lam)
-    else:
-        lam_v = dev_t(lam)
-    D_v = [dev_t(d) for d in D] if not is_scalar(D[0]) else [scalar_value(d) for d in D]
-    out = adjoint.solve_fixed(plan, phi_d, rhs_d, lam_v, D_v, cfg_kw)
+    lam_v = dev_t(lam).reshape(()) if is_scalar(lam) else dev_t(lam)
+    D_v = [dev_t(d).reshape(()) if is_scalar(d) else dev_t(d) for d in D]
+    Dxy_v = dev_t(D_xy) if D_xy is not None else None
+    out = adjoint.solve_fixed(plan, phi_d, rhs_d, lam_v, D_v, cfg_kw, Dxy_v)
     count = int(cfg_kw["max_iters_outer"])
     ret = (out, count)
     if return_info:

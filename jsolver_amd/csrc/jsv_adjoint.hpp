@@ -119,31 +119,115 @@ k_adj_restrict_apply(double* __restrict__ g, const double* __restrict__ rb,
                      const double* __restrict__ cy, const double* __restrict__ cz,
                      const double* __restrict__ cc, double* __restrict__ rbar,
                      double* __restrict__ bx, double* __restrict__ by, double* __restrict__ bz,
-                     double* __restrict__ bc, Geo gd, int dim, int first)
+                     double* __restrict__ bc, Geo gd, int dim, int first,
+                     const double* __restrict__ drr, const double* __restrict__ dlr,
+                     double* __restrict__ brr, double* __restrict__ blr)
 {
     const int z = (dim == 3) ? 2 + (int)blockIdx.z : 0;
     const int y = 2 + blockIdx.y * BY + threadIdx.y;
     const int x = 2 + blockIdx.x * BX + threadIdx.x;
     if (y > gd.my || x > gd.mx) return;
-    const long p = (long)z * gd.sz + (long)y * gd.sy + x;
+    const long sy = gd.sy;
+    const long p = (long)z * gd.sz + (long)y * sy + x;
     const double r = rb[p];
     double s = cc[p] * r;
     s -= cx[p] * rb[p + 1] + cx[p - 1] * rb[p - 1];
-    s -= cy[p] * rb[p + gd.sy] + cy[p - gd.sy] * rb[p - gd.sy];
+    s -= cy[p] * rb[p + sy] + cy[p - sy] * rb[p - sy];
     if (dim == 3) s -= cz[p] * rb[p + gd.sz] + cz[p - gd.sz] * rb[p - gd.sz];
+    if (drr != nullptr)   // 2D D_xy: the residual subtracts the diagonal couplings
+        s -= drr[p] * rb[p + sy + 1] + drr[p - sy - 1] * rb[p - sy - 1] + dlr[p] * rb[p + sy - 1] +
+             dlr[p - sy + 1] * rb[p - sy + 1];
     g[p] += s;
     if (((x + y + z) & 1) == first) {
         rbar[p] += r;
         bc[p] += r * phi[p];
         bx[p] -= r * phi[p + 1];
         bx[p - 1] -= r * phi[p - 1];
-        by[p] -= r * phi[p + gd.sy];
-        by[p - gd.sy] -= r * phi[p - gd.sy];
+        by[p] -= r * phi[p + sy];
+        by[p - sy] -= r * phi[p - sy];
         if (dim == 3) {
             bz[p] -= r * phi[p + gd.sz];
             bz[p - gd.sz] -= r * phi[p - gd.sz];
         }
+        if (drr != nullptr) {
+            // the couplings this point owns; the ones it shares with the
+            // first-half point (y-1, x-1) / (y-1, x+1) follow in
+            // k_adj_restrict_diag (that point writes these same entries)
+            brr[p] -= r * phi[p + sy + 1];
+            blr[p] -= r * phi[p + sy - 1];
+        }
     }
+}
+
+// Restriction adjoint with D_xy, second part over the first-half points: the
+// diagonal couplings to (y-1, x-1) and (y-1, x+1).
+__global__ void __launch_bounds__(BX * BY)
+k_adj_restrict_diag(const double* __restrict__ rb, const double* __restrict__ phi,
+                    double* __restrict__ brr, double* __restrict__ blr, Geo gd, int first)
+{
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    if (y > gd.my || x > gd.mx || ((x + y) & 1) != first) return;
+    const long sy = gd.sy;
+    const long p = (long)y * sy + x;
+    const double r = rb[p];
+    brr[p - sy - 1] -= r * phi[p - sy - 1];
+    blr[p - sy + 1] -= r * phi[p - sy + 1];
+}
+
+// ---- 2D D_xy smoother: one of the four sub-colours (lx, ly) -------------
+// Forward (rb2diag_point): phi_p = (sum_faces A phi_nb - rhs_p + sum_diag A phi_nb) / A_c
+// on the points x = (lx ? 2 : 3) + 2i, y = (ly ? 2 : 3) + 2j.  Kernel A over
+// those points (S: state after the sub-colour).
+__global__ void __launch_bounds__(BX * BY)
+k_adj_diag_a(double* __restrict__ g, double* __restrict__ t, const double* __restrict__ S,
+             const double* __restrict__ cc, double* __restrict__ rbar, double* __restrict__ bx,
+             double* __restrict__ by, double* __restrict__ bc, double* __restrict__ brr,
+             double* __restrict__ blr, Geo gd, int lx, int ly)
+{
+    const int y = (ly ? 2 : 3) + 2 * (int)(blockIdx.y * BY + threadIdx.y);
+    const int x = (lx ? 2 : 3) + 2 * (int)(blockIdx.x * BX + threadIdx.x);
+    if (y > gd.my || x > gd.mx) return;
+    const long sy = gd.sy;
+    const long p = (long)y * sy + x;
+    const double tp = g[p] / cc[p];
+    t[p] = tp;
+    rbar[p] -= tp;
+    bc[p] -= tp * S[p];
+    bx[p] += tp * S[p + 1];
+    bx[p - 1] += tp * S[p - 1];
+    by[p] += tp * S[p + sy];
+    by[p - sy] += tp * S[p - sy];
+    brr[p] += tp * S[p + sy + 1];
+    brr[p - sy - 1] += tp * S[p - sy - 1];
+    blr[p] += tp * S[p + sy - 1];
+    blr[p - sy + 1] += tp * S[p - sy + 1];
+    g[p] = 0.0;
+}
+
+// Kernel B over the unknowns outside the sub-colour: g_q += sum over q's
+// neighbours in the sub-colour of A(q, nb) t_nb (x neighbours when q shares
+// the sub-colour's row parity, y neighbours when it shares the column parity,
+// the diagonals when it shares neither).
+__global__ void __launch_bounds__(BX * BY)
+k_adj_diag_b(double* __restrict__ g, const double* __restrict__ t, const double* __restrict__ cx,
+             const double* __restrict__ cy, const double* __restrict__ drr,
+             const double* __restrict__ dlr, Geo gd, int lx, int ly)
+{
+    const int y = 2 + blockIdx.y * BY + threadIdx.y;
+    const int x = 2 + blockIdx.x * BX + threadIdx.x;
+    if (y > gd.my || x > gd.mx) return;
+    const bool xs = (x & 1) == (lx ? 0 : 1), ys = (y & 1) == (ly ? 0 : 1);
+    if (xs && ys) return;
+    const long sy = gd.sy;
+    const long p = (long)y * sy + x;
+    double s;
+    if (ys) s = cx[p] * t[p + 1] + cx[p - 1] * t[p - 1];
+    else if (xs) s = cy[p] * t[p + sy] + cy[p - sy] * t[p - sy];
+    else
+        s = drr[p] * t[p + sy + 1] + drr[p - sy - 1] * t[p - sy - 1] + dlr[p] * t[p + sy - 1] +
+            dlr[p - sy + 1] * t[p - sy + 1];
+    g[p] += s;
 }
 
 // Prolongation adjoint: g_c = P^T g over the coarse unknowns.
@@ -251,6 +335,44 @@ k_adj_inject_one(double* __restrict__ prev, const double* __restrict__ lev, Geo 
     const int z = (int)(i / ((long)g.nx * g.ny));
     if (x < 1 || x > g.mx + 1 || y < 1 || y > g.my + 1 || z < 1 || z > g.mz + 1) return;
     prev[(long)(2 * z - 1) * gp.sz + (long)(2 * y - 1) * gp.sy + (2 * x - 1)] += lev[i];
+}
+
+// 2D D_xy: bar of level l's injected D_xy from the bars of its coupling
+// arrays (k_setup2d: drr = (Dxy[y, x+1] + Dxy[y+1, x]) c, dlr = -(Dxy[y, x-1] +
+// Dxy[y+1, x]) c on the window [1, m+1]^2; solver_2d.py:411-416), gathered
+// per D_xy entry.
+__global__ void __launch_bounds__(256)
+k_adj_dxy_level(double* __restrict__ qbar, const double* __restrict__ brr,
+                const double* __restrict__ blr, Geo g, double c)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.n) return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    auto win = [&](int yy, int xx) { return yy >= 1 && yy <= g.my + 1 && xx >= 1 && xx <= g.mx + 1; };
+    double s = 0.0;
+    if (win(y, x - 1)) s += brr[i - 1];                       // Dxy[y, x] as the x+1 term of (y, x-1)
+    if (win(y - 1, x)) s += brr[i - g.sy] - blr[i - g.sy];    // as the y+1 term of (y-1, x)
+    if (win(y, x + 1)) s -= blr[i + 1];                       // as the x-1 term of (y, x+1)
+    qbar[i] = s * c;
+}
+
+// Sum of n doubles into *out (set, or added when accumulate): one workgroup,
+// fixed order, so repeated runs give the same bits.
+__global__ void __launch_bounds__(1024)
+k_sum_array(const double* __restrict__ a, long n, double* __restrict__ out, int accumulate)
+{
+    __shared__ double sh[16];
+    double v = 0.0;
+    for (long i = threadIdx.x; i < n; i += 1024) v += a[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s += sh[w];
+        *out = accumulate ? *out + s : s;
+    }
 }
 
 template <typename T>

@@ -101,6 +101,7 @@ struct jsv_plan {
     int m[3] = {0, 0, 0};          // x, y, z resolutions of level 0
     double idel[3] = {0, 0, 0};
     bool op_set = false, spatial = false, offdiag = false;
+    bool lam_array = false;         // lambda set from an array (else the scalar)
     Hier<double> h64;
     Hier<float> h32;
     double* phi_old = nullptr;      // level-0 copy for the distance
@@ -1410,6 +1411,7 @@ int jsv_plan_set_operator(jsv_plan* P, const double* lam, double lam_scalar, con
     if (!spatial && d_scalar == nullptr) return fail("scalar D values missing");
     if (dxy != nullptr && (dim != 2 || !spatial)) return fail("D_xy needs 2D array diffusion");
     P->spatial = spatial;
+    P->lam_array = lam != nullptr;
     ++P->op_version;                                      // the coarse program's image is stale
     if (P->offdiag != (dxy != nullptr)) drop_graphs(P);   // the smoother's kernels change
     P->offdiag = dxy != nullptr;
@@ -2095,6 +2097,8 @@ struct Adj {
     bool ready = false;
     int64_t bytes = 0;
     std::vector<double*> g, t, rb, rbar, bx, by, bz, bc, chain;
+    std::vector<double*> brr, blr;            // 2D D_xy coupling bars (allocated on first use)
+    double* sums = nullptr;                   // scalar cotangents, device double[4]
     std::vector<std::vector<double*>> pool;   // per level snapshot slots
     std::vector<size_t> used;
     std::vector<std::pair<int, double*>> tape;
@@ -2109,19 +2113,35 @@ struct Adj {
 };
 
 // snapshots per level for one cycle (mirrors mg_cycle)
+// snapshots per sweep: one per RBGS half, or one per sub-colour with D_xy
+long ops_per_sweep(const jsv_plan* P) { return (P->dim == 2 && P->offdiag) ? 4L : 2L; }
+
 void count_tape(const jsv_plan* P, const jsv_config& c, int l, bool fcyc, bool skip_pre,
                 std::vector<long>& cnt)
 {
+    const long w = ops_per_sweep(P);
     if (l == P->levels - 2) {
-        cnt[l] += 2L * (c.nu1 + c.nu2);
+        cnt[l] += w * (c.nu1 + c.nu2);
         return;
     }
-    if (!skip_pre) cnt[l] += 2L * c.nu1;
+    if (!skip_pre) cnt[l] += w * c.nu1;
     cnt[l] += 1;   // before the restriction
     if (fcyc)
         for (int i = 0; i < c.gamma; ++i) count_tape(P, c, l + 1, true, false, cnt);
     for (int i = 0; i < c.gamma; ++i) count_tape(P, c, l + 1, false, false, cnt);
-    cnt[l] += 2L * c.nu2;
+    cnt[l] += w * c.nu2;
+}
+
+// one device array of every level's size
+int one_level_set(jsv_plan* P, Adj& A, std::vector<double*>& v)
+{
+    const Hier<double>& H = P->h64;
+    v.assign(P->levels, nullptr);
+    for (int l = 0; l < P->levels; ++l) {
+        RETCHK(dalloc(P, (void**)&v[l], sizeof(double) * H.g[l].n));
+        A.bytes += (int64_t)sizeof(double) * H.g[l].n;
+    }
+    return 0;
 }
 
 int adj_alloc(jsv_plan* P, Adj& A, const jsv_config& c)
@@ -2148,12 +2168,18 @@ int adj_alloc(jsv_plan* P, Adj& A, const jsv_config& c)
         RETCHK(one(A.bc, true));
         RETCHK(one(A.chain, true));
         A.pool.assign(L, {});
+        RETCHK(dalloc(P, (void**)&A.sums, sizeof(double) * 4));
+        A.bytes += (int64_t)sizeof(double) * 4;
         A.ready = true;
+    }
+    if (P->dim == 2 && P->offdiag && A.brr.empty()) {
+        RETCHK(one_level_set(P, A, A.brr));
+        RETCHK(one_level_set(P, A, A.blr));
     }
     std::vector<long> cnt(L, 0);
     count_tape(P, c, 0, c.fcycle != 0, c.skip_pre != 0, cnt);
     // the optional pre-loop smoothing (skip_pre) also records on level 0
-    if (c.skip_pre) cnt[0] = cnt[0] > 2L * c.nu1 ? cnt[0] : 2L * c.nu1;
+    if (c.skip_pre) cnt[0] = cnt[0] > ops_per_sweep(P) * c.nu1 ? cnt[0] : ops_per_sweep(P) * c.nu1;
     for (int l = 0; l < L; ++l) {
         while ((long)A.pool[l].size() < cnt[l]) {
             double* p = nullptr;
@@ -2194,8 +2220,23 @@ int rec_half(jsv_plan* P, Adj& A, int l, int par, hipStream_t s)
     return 0;
 }
 
+// the reference's D_xy sub-colour order (solver_2d.py:686)
+constexpr int kDiagOrder[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+
 int rec_smooth(jsv_plan* P, Adj& A, int l, hipStream_t s)
 {
+    if (P->dim == 2 && P->offdiag) {
+        // each sub-colour, then its snapshot
+        Hier<double>& H = P->h64;
+        const Geo& g = H.g[l];
+        const dim3 gr((unsigned)((g.mx / 2 + jsv::BX - 1) / jsv::BX), (unsigned)((g.my / 2 + jsv::BY - 1) / jsv::BY), 1);
+        for (const auto& o : kDiagOrder) {
+            hipLaunchKernelGGL(jsv::k_rb2_diag<double>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
+                               H.cx[l], H.cy[l], H.cc[l], H.drr[l], H.dlr[l], g, o[0], o[1], (const int*)nullptr);
+            RETCHK(snap(P, A, l, s));
+        }
+        return 0;
+    }
     const int f = first_parity(P);
     RETCHK(rec_half(P, A, l, f, s));
     return rec_half(P, A, l, 1 - f, s);
@@ -2243,6 +2284,23 @@ int adj_half(jsv_plan* P, Adj& A, int l, int par, const double* S, hipStream_t s
 
 int adj_smooth(jsv_plan* P, Adj& A, int l, hipStream_t s)
 {
+    if (P->dim == 2 && P->offdiag) {
+        // the four sub-colours in reverse, each with the state after it
+        Hier<double>& H = P->h64;
+        const Geo& g = H.g[l];
+        const dim3 ga((unsigned)((g.mx / 2 + jsv::BX - 1) / jsv::BX), (unsigned)((g.my / 2 + jsv::BY - 1) / jsv::BY), 1);
+        const dim3 gb = grid_points(g, 2, g.mx - 1);
+        for (int k = 3; k >= 0; --k) {
+            const double* S = pop(A, l);
+            if (S == nullptr) return fail("adjoint tape mismatch at level %d", l);
+            const int lx = kDiagOrder[k][0], ly = kDiagOrder[k][1];
+            hipLaunchKernelGGL(jsv::k_adj_diag_a, ga, dim3(jsv::BX, jsv::BY), 0, s, A.g[l], A.t[l], S, H.cc[l],
+                               A.rbar[l], A.bx[l], A.by[l], A.bc[l], A.brr[l], A.blr[l], g, lx, ly);
+            hipLaunchKernelGGL(jsv::k_adj_diag_b, gb, dim3(jsv::BX, jsv::BY), 0, s, A.g[l], A.t[l], H.cx[l],
+                               H.cy[l], H.drr[l], H.dlr[l], g, lx, ly);
+        }
+        return 0;
+    }
     const int f = first_parity(P);
     double* sb = pop(A, l);
     double* sa = pop(A, l);
@@ -2279,9 +2337,15 @@ int adj_cycle(jsv_plan* P, Adj& A, const jsv_config& c, int l, bool fcyc, bool s
         dim3 gr = grid_points(gf, P->dim, gf.mx - 1);
         hipLaunchKernelGGL(jsv::k_adj_restrict_rbar, gr, dim3(jsv::BX, jsv::BY), 0, s, A.rb[l],
                            A.rbar[l + 1], gf, gc, P->dim, f);
+        const bool od = P->dim == 2 && P->offdiag;
         hipLaunchKernelGGL(jsv::k_adj_restrict_apply, gr, dim3(jsv::BX, jsv::BY), 0, s, A.g[l], A.rb[l],
                            S, H.cx[l], H.cy[l], P->dim == 3 ? H.cz[l] : nullptr, H.cc[l], A.rbar[l],
-                           A.bx[l], A.by[l], P->dim == 3 ? A.bz[l] : nullptr, A.bc[l], gf, P->dim, f);
+                           A.bx[l], A.by[l], P->dim == 3 ? A.bz[l] : nullptr, A.bc[l], gf, P->dim, f,
+                           od ? H.drr[l] : nullptr, od ? H.dlr[l] : nullptr, od ? A.brr[l] : nullptr,
+                           od ? A.blr[l] : nullptr);
+        if (od)
+            hipLaunchKernelGGL(jsv::k_adj_restrict_diag, gr, dim3(jsv::BX, jsv::BY), 0, s, A.rb[l], S, A.brr[l],
+                               A.blr[l], gf, f);
     }
     if (!skip_pre)
         for (int i = 0; i < c.nu1; ++i) RETCHK(adj_smooth(P, A, l, s));
@@ -2396,15 +2460,20 @@ int jsv_forward_checkpoints(jsv_plan* P, const double* phi0, const double* rhs, 
     return 0;
 }
 
-int jsv_adjoint(jsv_plan* P, const double* phi0, const double* rhs, const double* ckpt,
-                const jsv_config* c, const double* g_out, double* g_phi0, double* g_rhs,
-                double* g_lam, double* g_dxx, double* g_dyy, double* g_dzz, void* stream)
+int jsv_adjoint_full(jsv_plan* P, const double* phi0, const double* rhs, const double* ckpt,
+                     const jsv_config* c, const double* g_out, double* g_phi0, double* g_rhs,
+                     double* g_lam, double* g_dxx, double* g_dyy, double* g_dzz, double* g_dxy,
+                     double* g_scalar, void* stream)
 {
     if (P == nullptr || !P->op_set) return fail("plan missing or operator not set");
     if (phi0 == nullptr || rhs == nullptr || ckpt == nullptr || g_out == nullptr)
         return fail("NULL argument");
-    if (P->offdiag) return fail("the reverse pass does not support D_xy");
     if (P->batch != 1) return fail("the reverse pass needs an unbatched plan");
+    if (g_lam != nullptr && !P->lam_array)
+        return fail("lambda is a scalar on this plan: its cotangent goes to g_scalar[0]");
+    if ((g_dxx != nullptr || g_dyy != nullptr || g_dzz != nullptr) && !P->spatial)
+        return fail("D is scalar on this plan: its cotangents go to g_scalar[1..3]");
+    if (g_dxy != nullptr && !(P->dim == 2 && P->offdiag)) return fail("the plan's operator has no D_xy");
     RETCHK(check_cfg(c));
     hipStream_t s = (hipStream_t)stream;
     Hier<double>& H = P->h64;
@@ -2422,6 +2491,10 @@ int jsv_adjoint(jsv_plan* P, const double* phi0, const double* rhs, const double
         RETCHK(zero(A.t[l], nl, s));
         RETCHK(zero(A.rb[l], nl, s));
         RETCHK(zero(A.g[l], nl, s));
+        if (!A.brr.empty()) {
+            RETCHK(zero(A.brr[l], nl, s));
+            RETCHK(zero(A.blr[l], nl, s));
+        }
     }
     HIPCHK(hipMemcpyAsync(H.rhs[0], rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(A.g[0], g_out, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -2446,6 +2519,41 @@ int jsv_adjoint(jsv_plan* P, const double* phi0, const double* rhs, const double
     if (g_rhs) HIPCHK(hipMemcpyAsync(g_rhs, A.rbar[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     // ---- coefficient chain ------------------------------------------------
     const Geo& g0 = H.g[0];
+    // scalar parts: lambda enters every level's diagonal; a scalar D_a every
+    // level's a-faces (AMat_x = D_xx idel_x^2, AMat_cc = 2 (AMat_x + AMat_y) +
+    // lambda, solver_2d.py:417-421): its cotangent is the sum of each level's
+    // total face bar (face bar plus the bars of the two diagonals it enters)
+    if (g_scalar != nullptr) {
+        RETCHK(zero(A.sums, 4, s));
+        if (!P->lam_array)
+            for (int l = 0; l < L; ++l)
+                hipLaunchKernelGGL(jsv::k_sum_array, dim3(1), dim3(1024), 0, s, A.bc[l], H.g[l].n, A.sums, 1);
+        if (!P->spatial) {
+            std::vector<double*> bface[3] = {A.bx, A.by, A.bz};
+            for (int a = 0; a < P->dim; ++a)
+                for (int l = 0; l < L; ++l) {
+                    const double id = P->idel[a] * std::ldexp(1.0, -l);
+                    const Geo& g = H.g[l];
+                    const long step = a == 0 ? 1 : a == 1 ? g.sy : g.sz;
+                    hipLaunchKernelGGL(jsv::k_adj_face_total, grid_exact(g.n), dim3(256), 0, s, A.t[l],
+                                       bface[a][l], A.bc[l], g, step, id * id, 0);
+                    hipLaunchKernelGGL(jsv::k_sum_array, dim3(1), dim3(1024), 0, s, A.t[l], g.n, A.sums + 1 + a, 1);
+                }
+        }
+        HIPCHK(hipMemcpyAsync(g_scalar, A.sums, sizeof(double) * 4, hipMemcpyDeviceToDevice, s));
+    }
+    if (g_dxy != nullptr) {
+        // D_xy is injected (not averaged) level to level, like lambda
+        RETCHK(zero(g_dxy, n, s));
+        for (int l = 0; l < L; ++l) {
+            const double idx = P->idel[0] * std::ldexp(1.0, -l), idy = P->idel[1] * std::ldexp(1.0, -l);
+            const Geo& g = H.g[l];
+            hipLaunchKernelGGL(jsv::k_adj_dxy_level, grid_exact(g.n), dim3(256), 0, s, A.chain[l], A.brr[l],
+                               A.blr[l], g, idx * idy * 0.25);
+            hipLaunchKernelGGL(jsv::k_adj_inject_to_fine, grid_exact(g.n), dim3(256), 0, s, g_dxy, A.chain[l], g,
+                               g0, 1 << l, 2);
+        }
+    }
     if (g_lam) {
         RETCHK(zero(g_lam, n, s));
         for (int l = 0; l < L; ++l)
@@ -2488,6 +2596,14 @@ int jsv_adjoint(jsv_plan* P, const double* phi0, const double* rhs, const double
     }
     LAUNCHCHK();
     return 0;
+}
+
+int jsv_adjoint(jsv_plan* P, const double* phi0, const double* rhs, const double* ckpt,
+                const jsv_config* c, const double* g_out, double* g_phi0, double* g_rhs,
+                double* g_lam, double* g_dxx, double* g_dyy, double* g_dzz, void* stream)
+{
+    return jsv_adjoint_full(P, phi0, rhs, ckpt, c, g_out, g_phi0, g_rhs, g_lam, g_dxx, g_dyy, g_dzz, nullptr,
+                            nullptr, stream);
 }
 
 int jsv_adjoint_device_bytes(const jsv_plan* P, int64_t* bytes)
