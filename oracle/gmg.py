@@ -78,8 +78,13 @@ class OracleGrid:
 # do_BCs (jsolver/boundary_handler_2d.py:3-29, boundary_handler_3d.py:3-35)
 # ----------------------------------------------------------------------------
 def _ft(*arrs):
-    """float64, or complex128 when any input is complex (complex-step JVPs)."""
-    return np.result_type(np.float64, *[np.asarray(a).dtype for a in arrs])
+    """float64; float32 when every input is float32 (the FP32 restatement of
+    the mixed solver's inner cycles, see cast_operator); complex128 when any
+    input is complex (complex-step JVPs)."""
+    dts = [np.asarray(a).dtype for a in arrs]
+    if dts and all(d == np.float32 for d in dts):
+        return np.dtype(np.float32)
+    return np.result_type(np.float64, *dts)
 
 
 def zero_rim(a: np.ndarray, rim: int = 1) -> np.ndarray:
@@ -196,6 +201,28 @@ class Operator:
     lv: List[Level] = field(default_factory=list)
     off_diag: bool = False
     dtype: object = np.float64     # complex128 when a coefficient input is complex
+
+
+def cast_operator(op: "Operator", dtype=np.float32) -> "Operator":
+    """The operator with every coefficient rounded to `dtype`: what the mixed
+    solver's FP32 hierarchy holds (coefficients set up in FP64, then cast).
+    With float32 phi and rhs, smooth / restrict / prolong / cycle then run in
+    float32, one rounding per operation in the same order as in FP64."""
+    def c(v):
+        if v is None:
+            return None
+        if _is_scalar(v):
+            return dtype(v)
+        return np.asarray(v).astype(dtype)
+    lv = []
+    for L in op.lv:
+        lv.append(Level(shape=L.shape,
+                        a_minus=None if L.a_minus is None else [c(a) for a in L.a_minus],
+                        a_plus=None if L.a_plus is None else [c(a) for a in L.a_plus],
+                        a_c=c(L.a_c),
+                        a_axis=None if L.a_axis is None else [c(a) for a in L.a_axis],
+                        a_diag=None if L.a_diag is None else {k: c(v) for k, v in L.a_diag.items()}))
+    return Operator(dim=op.dim, levels=op.levels, lv=lv, off_diag=op.off_diag, dtype=np.dtype(dtype))
 
 
 def _inject(prev: np.ndarray, shape) -> np.ndarray:
