@@ -83,8 +83,9 @@ def generate(samples, m, batch, seed, label_rtol, dev, log=print, gamma=1, max_c
     return torch.cat(X), torch.cat(Y), np.concatenate(counts)
 
 
-def train(net, X, Y, epochs, batch, lr, log=print, seed=0):
-    """Adam on MSE of the normalised output; all kernels hand-written."""
+def train(net, X, Y, epochs, batch, lr, log=print, seed=0, on_epoch=None):
+    """Adam on MSE of the normalised output; all kernels hand-written.
+    on_epoch(epoch, train_loss) runs after every epoch (learning curves)."""
     S = X.shape[0]
     gen = torch.Generator(device=X.device)
     gen.manual_seed(seed)
@@ -115,6 +116,8 @@ def train(net, X, Y, epochs, batch, lr, log=print, seed=0):
                 log(f"[train] epoch {ep} step {step}/{total} loss {run / (j + 1):.4e} "
                     f"({time.perf_counter() - t0:.1f}s)")
         hist.append(run / max(1, steps_per_epoch))
+        if on_epoch is not None:
+            on_epoch(ep, hist[-1])
     return hist
 
 
@@ -215,7 +218,8 @@ def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_
     torch.cuda.synchronize()
     t_warm = time.perf_counter() - t0
     good = np.isfinite(r0) & (r0 <= rtol) & np.isfinite(r1) & (r1 <= rtol)
-    c0, c1 = np.asarray(c0)[good], np.asarray(c1)[good]
+    c0_all, c1_all = np.asarray(c0), np.asarray(c1)
+    c0, c1 = c0_all[good], c1_all[good]
     res = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol, "guess_sweeps": guess_sweeps, "criterion": criterion,
            "unconverged_excluded": int((~good).sum()),
            "cycles_zero_mean": float(np.mean(c0)), "cycles_warm_mean": float(np.mean(c1)),
@@ -224,9 +228,148 @@ def evaluate(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_
            "time_zero_s": t_zero, "time_warm_s": t_warm,
            "guess_criterion_ratio_median": float(np.median(rel0)),
            "fallbacks": int((~use).sum().item()),
-           "max_final_rel_residual": float(max(np.max(r0[good]), np.max(r1[good])))}
-    log(f"[eval] {json.dumps(res)}")
+           "max_final_rel_residual": float(max(np.max(r0[good]), np.max(r1[good]))),
+           "per_problem": {"criterion_ratio": [float(v) for v in np.asarray(rel0)],
+                           "used": [bool(v) for v in use.cpu().numpy()],
+                           "converged": [bool(v) for v in good],
+                           "cycles_zero": [int(v) for v in np.asarray(c0_all)],
+                           "cycles_warm": [int(v) for v in np.asarray(c1_all)]}}
+    log(f"[eval] {json.dumps({k: v for k, v in res.items() if k != 'per_problem'})}")
     return res
+
+
+def _rel_err(a, b):
+    """Per-problem ||a - b|| / ||b|| over the unknowns of (B, m+3, m+3) arrays."""
+    sl = (slice(None), slice(2, -2), slice(2, -2))
+    d = (a[sl] - b[sl]).flatten(1).norm(dim=1)
+    return (d / b[sl].flatten(1).norm(dim=1).clamp_min(1e-300)).cpu().numpy()
+
+
+def _criteria(g, phi0, rhs, lam, dxx, dyy, gamma, rtol):
+    """The two fallback criteria of a batch of starts: ||rhs - L phi0|| / ||rhs||
+    ("rhs") and the relative residual after one cycle from phi0 ("cycle")."""
+    res0 = dict(fixed=True, max_iters_outer=0, stop="rel_residual", rtol=rtol, precision="mixed")
+    _, _, r_rhs = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, **res0)
+    p1, _, _ = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, fixed=True, max_iters_outer=1, gamma=gamma)
+    _, _, r_cyc = solve_batched_2d(g, g, p1, rhs, lam, dxx, dyy, **res0)
+    return np.asarray(r_rhs), np.asarray(r_cyc)
+
+
+def diagnose(net, m, B, seed, m_train, dev, rtol=1e-10, gamma=1, log=print, max_cycles=50000,
+             n_after=50, guess_sweeps=4):
+    """What a warm start can gain on this problem class, and why the fallback
+    rule decides as it does.  For the network's guess (raw and after
+    `guess_sweeps` RBGS sweeps) and for reference guesses made from the
+    converged solution phi* (phi* rounded to bf16: the best a bf16-output
+    network can give; phi* plus a smooth relative error of 1e-3, 1e-2, 1e-1),
+    per problem:
+      * both fallback criteria (relative to the zero start's: > 1 means the
+        guess falls back);
+      * cycles to rtol when the guess is used regardless of the criteria;
+      * the error ||phi_N - phi*|| / ||phi*|| after n_after cycles, against
+        the zero start's (the error left in the slowly converging modes)."""
+    g = grid_1D(m)
+    lam, dxx, dyy, rhs = make_problems(B, m, seed, dev)
+    kw = dict(gamma=gamma, stop="rel_residual", rtol=rtol, max_iters_outer=max_cycles)
+    zero = torch.zeros_like(rhs)
+    star, _, rs = solve_batched_2d(g, g, zero, rhs, lam, dxx, dyy, gamma=gamma, stop="rel_residual",
+                                   rtol=min(rtol, 1e-13), max_iters_outer=max_cycles)
+    _, c0, r0 = solve_batched_2d(g, g, zero, rhs, lam, dxx, dyy, **kw)
+    pz, _, _ = solve_batched_2d(g, g, zero, rhs, lam, dxx, dyy, fixed=True, max_iters_outer=n_after, gamma=gamma)
+    e_zero = _rel_err(pz, star)
+    crit_zero = _criteria(g, zero, rhs, lam, dxx, dyy, gamma, rtol)
+    guesses = {}
+    phi_net = guess_phi(net, lam, dxx, dyy, rhs, m_train)
+    guesses["net"] = phi_net
+    if guess_sweeps > 0:
+        guesses[f"net_sweeps{guess_sweeps}"] = smooth_guess(phi_net, rhs, lam, dxx, dyy, guess_sweeps)
+    guesses["exact_bf16"] = star.to(torch.bfloat16).double()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed) + 17)
+    pert = grf_batch(B, m + 3, g.del_, torch.full((B,), 0.1, device=dev, dtype=torch.float64), gen, dev)
+    sl = (slice(None), slice(2, -2), slice(2, -2))
+    rms = star[sl].flatten(1).norm(dim=1) / math.sqrt((m - 1) ** 2)
+    for eps in (1e-3, 1e-2, 1e-1):
+        gs = star + eps * rms[:, None, None] * pert
+        gz = torch.zeros_like(gs)
+        gz[sl] = gs[sl]
+        guesses[f"exact_smooth_{eps:g}"] = gz
+    out = {"m": m, "batch": B, "gamma": gamma, "rtol": rtol, "n_after": n_after,
+           "cycles_zero_mean": float(np.mean(c0)), "error_after_n_zero_median": float(np.median(e_zero)),
+           "unconverged_zero": int(np.sum(~(np.asarray(r0) <= rtol))), "guesses": {}}
+    for name, phi0 in guesses.items():
+        crit = _criteria(g, phi0, rhs, lam, dxx, dyy, gamma, rtol)
+        _, c1, r1 = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, **kw)
+        pn, _, _ = solve_batched_2d(g, g, phi0, rhs, lam, dxx, dyy, fixed=True, max_iters_outer=n_after,
+                                    gamma=gamma)
+        ok = (np.asarray(r0) <= rtol) & (np.asarray(r1) <= rtol)
+        c0k, c1k = np.asarray(c0)[ok], np.asarray(c1)[ok]
+        ratio_rhs = crit[0] / np.maximum(crit_zero[0], 1e-300)
+        ratio_cyc = crit[1] / np.maximum(crit_zero[1], 1e-300)
+        e_g = _rel_err(pn, star)
+        out["guesses"][name] = {
+            "guess_rel_error_median": float(np.median(_rel_err(phi0, star))),
+            "rhs_criterion_ratio_median": float(np.median(ratio_rhs)),
+            "cycle_criterion_ratio_median": float(np.median(ratio_cyc)),
+            "fallbacks_rhs": int(np.sum(ratio_rhs > 1.0)),
+            "fallbacks_cycle": int(np.sum(ratio_cyc > 1.0)),
+            "cycles_used_anyway_mean": float(np.mean(c1k)) if c1k.size else None,
+            "cycle_reduction_used_anyway": float(1.0 - np.sum(c1k) / np.sum(c0k)) if c0k.size else None,
+            "error_after_n_median": float(np.median(e_g)),
+            "error_after_n_vs_zero_median": float(np.median(e_g / np.maximum(e_zero, 1e-300))),
+            "unconverged": int(np.sum(~ok)),
+        }
+        log(f"[diag] {name}: {json.dumps(out['guesses'][name])}")
+    return out
+
+
+class CurveProbe:
+    """Learning-curve measurements after every epoch: normalised MSE on
+    held-out samples, and on a held-out batch of problems at the training
+    resolution the cycles to rtol from the network's guess used regardless of
+    the fallback rule, and its error after n_after cycles against the zero
+    start's."""
+
+    def __init__(self, m, B, seed, dev, rtol, gamma, n_after, label_rtol, max_cycles, heldout, log):
+        self.g = grid_1D(m)
+        self.m, self.dev, self.rtol, self.gamma, self.n_after = m, dev, rtol, gamma, n_after
+        self.max_cycles, self.log = max_cycles, log
+        self.prob = make_problems(B, m, seed, dev)
+        lam, dxx, dyy, rhs = self.prob
+        zero = torch.zeros_like(rhs)
+        self.star, _, _ = solve_batched_2d(self.g, self.g, zero, rhs, lam, dxx, dyy, gamma=gamma,
+                                           stop="rel_residual", rtol=1e-13, max_iters_outer=max_cycles)
+        _, self.c0, _ = solve_batched_2d(self.g, self.g, zero, rhs, lam, dxx, dyy, gamma=gamma,
+                                         stop="rel_residual", rtol=rtol, max_iters_outer=max_cycles)
+        pz, _, _ = solve_batched_2d(self.g, self.g, zero, rhs, lam, dxx, dyy, fixed=True,
+                                    max_iters_outer=n_after, gamma=gamma)
+        self.e_zero = _rel_err(pz, self.star)
+        self.Xh, self.Yh, _ = generate(heldout, m, 64, seed + 1, label_rtol, dev, lambda *_: None, gamma,
+                                       max_cycles)
+        self.points = []
+
+    def __call__(self, net, epoch, train_loss):
+        lam, dxx, dyy, rhs = self.prob
+        mse, n = 0.0, 0
+        for i in range(0, self.Xh.shape[0], 16):
+            x = ((self.Xh[i:i + 16] - net.in_shift.to(self.dev)) / net.in_scale.to(self.dev)).to(torch.bfloat16)
+            out = net.forward(x.contiguous())
+            y = self.Yh[i:i + 16] / net.out_scale
+            mse += float(((out - y) ** 2).sum())
+            n += y.numel()
+        phi0 = guess_phi(net, lam, dxx, dyy, rhs, self.m)
+        _, c1, r1 = solve_batched_2d(self.g, self.g, phi0, rhs, lam, dxx, dyy, gamma=self.gamma,
+                                     stop="rel_residual", rtol=self.rtol, max_iters_outer=self.max_cycles)
+        pn, _, _ = solve_batched_2d(self.g, self.g, phi0, rhs, lam, dxx, dyy, fixed=True,
+                                    max_iters_outer=self.n_after, gamma=self.gamma)
+        e = _rel_err(pn, self.star)
+        pt = {"epoch": epoch + 1, "train_loss": train_loss, "heldout_nmse": mse / n,
+              "guess_rel_error_median": float(np.median(_rel_err(phi0, self.star))),
+              "cycles_zero_mean": float(np.mean(self.c0)), "cycles_guess_used_anyway_mean": float(np.mean(c1)),
+              "cycle_reduction_used_anyway": float(1.0 - np.sum(c1) / np.sum(self.c0)),
+              "error_after_n_vs_zero_median": float(np.median(e / np.maximum(self.e_zero, 1e-300)))}
+        self.points.append(pt)
+        self.log(f"[curve] {json.dumps(pt)}")
 
 
 def main(argv=None):
@@ -244,6 +387,11 @@ def main(argv=None):
     ap.add_argument("--gamma", type=int, default=1)
     ap.add_argument("--max-cycles", type=int, default=50000)
     ap.add_argument("--guess-sweeps", type=int, default=4)
+    ap.add_argument("--curve", type=int, default=0,
+                    help="problems of the per-epoch learning-curve batch (0: no curve)")
+    ap.add_argument("--curve-heldout", type=int, default=256, help="held-out samples for the curve's MSE")
+    ap.add_argument("--diagnose", type=int, default=0,
+                    help="problems at --eval-m for the gain diagnosis (0: none)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--eval-only", default=None, help="state file of a trained network: evaluate it only")
     a = ap.parse_args(argv)
@@ -251,15 +399,23 @@ def main(argv=None):
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+
+    def evaluate_all(net):
+        ev = {}
+        for criterion in ("rhs", "cycle"):
+            for sw in sorted({0, a.guess_sweeps}):
+                ev[f"{criterion}_sweeps{sw}"] = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev,
+                                                         gamma=a.gamma, log=log, max_cycles=a.max_cycles,
+                                                         guess_sweeps=sw, criterion=criterion)
+        return ev
+
     if a.eval_only:
         net = UNet(seed=a.seed, device=dev)
         net.load_state_dict(torch.load(a.eval_only, weights_only=True))
-        out = {}
-        for criterion in ("rhs", "cycle"):
-            for sw in sorted({0, a.guess_sweeps}):
-                out[f"{criterion}_sweeps{sw}"] = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev, gamma=a.gamma,
-                                                      log=log, max_cycles=a.max_cycles, guess_sweeps=sw,
-                                                      criterion=criterion)
+        out = {"eval": evaluate_all(net)}
+        if a.diagnose:
+            out["diagnosis"] = diagnose(net, a.eval_m, a.diagnose, 999_000, a.m, dev, gamma=a.gamma, log=log,
+                                        max_cycles=a.max_cycles, guess_sweeps=a.guess_sweeps)
         print(json.dumps(out), flush=True)
         return
     t0 = time.perf_counter()
@@ -268,23 +424,29 @@ def main(argv=None):
     X_kept = X.shape[0]
     net = UNet(seed=a.seed, device=dev)
     fit_normalisation(net, X, Y)
+    probe = None
+    if a.curve:
+        probe = CurveProbe(a.m, a.curve, 555_000, dev, 1e-10, a.gamma, 50, a.label_rtol, a.max_cycles,
+                           a.curve_heldout, log)
     t0 = time.perf_counter()
-    hist = train(net, X, Y, a.epochs, a.batch, a.lr, log)
+    hist = train(net, X, Y, a.epochs, a.batch, a.lr, log,
+                 on_epoch=(lambda ep, loss: probe(net, ep, loss)) if probe else None)
     t_train = time.perf_counter() - t0
     if a.out:
         torch.save(net.state_dict(), a.out)
     del X, Y
     torch.cuda.empty_cache()
-    ev = {}
-    for sw in sorted({0, a.guess_sweeps}):
-        ev[f"sweeps{sw}"] = evaluate(net, a.eval_m, a.eval_batch, 777_000, a.m, dev, gamma=a.gamma, log=log,
-                                     max_cycles=a.max_cycles, guess_sweeps=sw)
-    ev_train_res = evaluate(net, a.m, a.eval_batch, 888_000, a.m, dev, gamma=a.gamma, log=log,
-                            max_cycles=a.max_cycles, guess_sweeps=a.guess_sweeps)
-    print(json.dumps({"samples": a.samples, "kept": int(X_kept), "m": a.m, "gamma": a.gamma,
-                      "data_s": t_data, "train_s": t_train,
-                      "label_cycles_mean": float(np.mean(counts)), "loss_per_epoch": hist,
-                      "eval": ev, "eval_train_resolution": ev_train_res}), flush=True)
+    out = {"samples": a.samples, "kept": int(X_kept), "m": a.m, "gamma": a.gamma, "epochs": a.epochs,
+           "data_s": t_data, "train_s": t_train, "label_cycles_mean": float(np.mean(counts)),
+           "loss_per_epoch": hist}
+    if probe is not None:
+        out["curve"] = probe.points
+    if a.eval_batch > 0:
+        out["eval"] = evaluate_all(net)
+    if a.diagnose:
+        out["diagnosis"] = diagnose(net, a.eval_m, a.diagnose, 999_000, a.m, dev, gamma=a.gamma, log=log,
+                                    max_cycles=a.max_cycles, guess_sweeps=a.guess_sweeps)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
