@@ -85,23 +85,32 @@ def roofline(a, grids, m):
 
 
 def _stream_bandwidth():
-    """Bandwidth this GPU sustains on a plain streaming kernel (torch's
-    elementwise y = 1.0 * x over 1 GiB of FP32: read + write bytes / time):
-    context for the roofline fraction, whose peak is the published 8 TB/s."""
+    """Bandwidth this GPU sustains on the library's own streaming kernel
+    (jsv_stream_bench: 16-byte non-temporal loads and stores over 256 MiB
+    arrays): a copy (1 read : 1 write) and the RBGS sweep's ratio (6 reads :
+    1 write).  Context for the roofline fraction, whose peak is the published
+    8 TB/s."""
     import torch
-    n = 1 << 28
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    torch.mul(src, 1.0, out=dst)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(10):
-        torch.mul(src, 1.0, out=dst)
-    e1.record()
-    torch.cuda.synchronize()
-    t = e0.elapsed_time(e1) / 1e3 / 10
-    del src, dst
-    return round(2 * 4 * n / t / 1e9, 1)
+
+    from jsolver_amd import _lib
+    from jsolver_amd._device import ptr, stream_ptr
+    lib = _lib.load()
+    n = 1 << 26
+    out = {}
+    for k in (1, 6):
+        src = torch.ones(k * n, dtype=torch.float32, device="cuda")
+        dst = torch.empty(n, dtype=torch.float32, device="cuda")
+        _lib.check(lib.jsv_stream_bench(ptr(src), k, n, ptr(dst), 0, stream_ptr()))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            _lib.check(lib.jsv_stream_bench(ptr(src), k, n, ptr(dst), 0, stream_ptr()))
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / 10
+        out[f"read{k}_write1"] = round(4 * n * (k + 1) / t / 1e9, 1)
+        del src, dst
+    return out
 
 
 def _traffic():

@@ -226,6 +226,14 @@ int jsv_max_abs_diff(const double *a, const double *b, int64_t n, double *out, v
 int jsv_max_abs_diff_noalloc(const double *a, const double *b, int64_t n, int64_t nseg, double *out,
                              void *stream);
 
+/* Streaming-bandwidth reference, no counterpart in the reference: dst[i] =
+ * sum over a < k of src[a*n + i] (k = 1, 2 or 6 source arrays of n floats,
+ * consecutive in src; 16-byte aligned; n a multiple of 4) with 16-byte
+ * non-temporal loads and stores.  k = 6 reads and writes in the RBGS sweep's
+ * 6:1 ratio.  blocks: workgroups of 256 threads (0 = one per 1024 floats).
+ * Bytes moved per launch: 4 n (k + 1). */
+int jsv_stream_bench(const float *src, int k, int64_t n, float *dst, int blocks, void *stream);
+
 /* ---- boundary handler (do_BCs, boundary_handler_2d.py:3-29 /
  * boundary_handler_3d.py:3-35), in place on a caller array.  vals_* may be
  * NULL (zero).  3D: like the reference, the z faces take vals_yb/vals_ye. ---- */

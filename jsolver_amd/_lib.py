@@ -93,6 +93,7 @@ SIGNATURES = {
     "jsv_residual": (_I, [_P, _P, _P, _P, _DP, _P]),
     "jsv_max_abs_diff": (_I, [_P, _P, _I64, _DP, _P]),
     "jsv_max_abs_diff_noalloc": (_I, [_P, _P, _I64, _I64, _DP, _P]),
+    "jsv_stream_bench": (_I, [_P, _I, _I64, _P, _I, _P]),
     "jsv_dirichlet_lift": (_I, [_P, _P, _P, _P, _P]),
     "jsv_do_bcs_2d": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "jsv_do_bcs_3d": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),

@@ -1723,6 +1723,27 @@ int jsv_max_abs_diff_noalloc(const double* a, const double* b, int64_t n, int64_
     return 0;
 }
 
+int jsv_stream_bench(const float* src, int k, int64_t n, float* dst, int blocks, void* stream)
+{
+    if (src == nullptr || dst == nullptr) return fail("NULL argument");
+    if (n <= 0 || n % 4 != 0) return fail("n must be a positive multiple of 4 (got %lld)", (long long)n);
+    if (((uintptr_t)src | (uintptr_t)dst) % 16 != 0) return fail("arrays must be 16-byte aligned");
+    const long n4 = (long)(n / 4);
+    long nb = blocks > 0 ? blocks : (n4 + 255) / 256;
+    if (nb > 65535L * 16) nb = 65535L * 16;
+    const jsv::f32x4* s4 = reinterpret_cast<const jsv::f32x4*>(src);
+    jsv::f32x4* d4 = reinterpret_cast<jsv::f32x4*>(dst);
+    hipStream_t s = (hipStream_t)stream;
+    switch (k) {
+    case 1: hipLaunchKernelGGL(jsv::k_stream<1>, dim3((unsigned)nb), dim3(256), 0, s, s4, d4, n4); break;
+    case 2: hipLaunchKernelGGL(jsv::k_stream<2>, dim3((unsigned)nb), dim3(256), 0, s, s4, d4, n4); break;
+    case 6: hipLaunchKernelGGL(jsv::k_stream<6>, dim3((unsigned)nb), dim3(256), 0, s, s4, d4, n4); break;
+    default: return fail("k must be 1, 2 or 6 (got %d)", k);
+    }
+    LAUNCHCHK();
+    return 0;
+}
+
 static int do_bcs(double* arr, int dim, int nz, int ny, int nx, int rim, int direction,
                   const double* vxb, const double* vxe, const double* vyb, const double* vye,
                   void* stream)

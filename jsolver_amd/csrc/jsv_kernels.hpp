@@ -824,9 +824,14 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         const int zb = zs - 1;
         if (inner && zb >= z0 && zb < z1) {
             T* orow = pout + ((long)zb * sz + (long)y * sy);
+            // both colours of the thread's pairs leave as one vector store; a
+            // pair reaching past the last unknown (x = mx + 1, a boundary
+            // node) writes the zero that node holds
+            PairVec<T, 2 * NP> ov;
 #pragma unroll
             for (int pp = 0; pp < NP; ++pp) {
                 const int xr = 2 * (k + pp) + e;
+                T blk = (T)0;
                 if (xr <= g.mx) {
                     const T r_zp = Rv(zb + 1, y, xr), r_zm = Rv(zb - 1, y, xr);
                     const T r_yp = Rv(zb, y + 1, xr), r_ym = Rv(zb, y - 1, xr);
@@ -837,10 +842,13 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                     acc = acc + H(pp, 4) * r_xp;
                     acc = acc + H(pp, 5) * r_xm;
                     acc = acc - H(pp, 6);
-                    orow[xr] = acc / H(pp, 7);
+                    blk = acc / H(pp, 7);
                 }
-                if ((xr ^ 1) <= g.mx) orow[xr ^ 1] = red_prev[pp];
+                const T rp = ((xr ^ 1) <= g.mx) ? red_prev[pp] : (T)0;
+                ov.v[2 * pp] = e ? rp : blk;
+                ov.v[2 * pp + 1] = e ? blk : rp;
             }
+            *reinterpret_cast<PairVec<T, 2 * NP>*>(orow + 2 * k) = ov;
         }
         // ---- keep plane zs's black halves (the points that are not red)
 #pragma unroll
@@ -1899,6 +1907,25 @@ __global__ void __launch_bounds__(256) k_fill(T* __restrict__ a, long n, T v)
 __global__ void __launch_bounds__(64) k_set_u64(unsigned long long* a, unsigned long long v)
 {
     if (threadIdx.x == 0) a[0] = v;
+}
+
+// Streaming bandwidth reference: dst[i] = sum over the K source arrays
+// (consecutive in src, n floats each) of src[a n + i], as 16-byte vectors.
+// K = 1 is a copy; K = 6 reads and writes in the RBGS sweep's ratio (phi,
+// rhs, three faces and the diagonal read; phi written).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <int K>
+__global__ void __launch_bounds__(256) k_stream(const f32x4* __restrict__ src, f32x4* __restrict__ dst, long n4)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long)gridDim.x * 256) {
+        f32x4 v[K];
+#pragma unroll
+        for (int a = 0; a < K; ++a) v[a] = __builtin_nontemporal_load(src + (long)a * n4 + i);
+        f32x4 s = v[0];
+#pragma unroll
+        for (int a = 1; a < K; ++a) s = s + v[a];
+        __builtin_nontemporal_store(s, dst + i);
+    }
 }
 
 template <typename TS, typename TD>
