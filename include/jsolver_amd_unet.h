@@ -15,12 +15,14 @@ extern "C" {
 
 const char *jsv_unet_last_error(void);
 /* y = act(conv(x, wt) + bias), 3x3 (padding 1) or 1x1; x [N][H][W][Cin] bf16,
- * wt [Cout][Kp] bf16 (k = tap * Cin + ci, zero padded to Kp, a multiple of 16),
- * bias fp32 (nullable); outputs bf16 y16 and/or fp32 y32 [N][H][W][Cout]. */
+ * wt [Cout][Kp] bf16 (k = tap * Cin + ci, zero padded to Kp, a multiple of 32),
+ * bias fp32 (nullable); outputs bf16 y16 and/or fp32 y32 [N][H][W][Cout];
+ * Cin a multiple of 4, Cout <= 128. */
 int jsv_conv_fwd(const void *x, const void *wt, const float *bias, void *y16, float *y32, int N,
                  int H, int W, int Cin, int Cout, int ksize, int Kp, int relu, void *stream);
 /* dw [Cout][Kp] = sum over pixels of dy x im2col(x); db [Cout] = sum dy (nullable);
- * part: chunks * Cout * Kp floats of workspace (deterministic two-pass sum). */
+ * part: chunks * Cout * (Kp + 1) floats of workspace (deterministic two-pass
+ * sum); Cout in [4, 64] and Cin <= 128, multiples of 4. */
 int jsv_conv_wgrad(const void *x, const void *dy, float *dw, float *db, float *part, int chunks,
                    int N, int H, int W, int Cin, int Cout, int ksize, int Kp, void *stream);
 /* 2x2 average pooling (backward = 1: x is dy of the pooled map, y is dx) */

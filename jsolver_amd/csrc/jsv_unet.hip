@@ -4,15 +4,15 @@
 // Activations are NHWC bf16.  A 3x3 (or 1x1) convolution is an implicit GEMM
 //   out[m][co] = sum_k A(m, k) * Wt[co][k],   m = (n, h, w),  k = (tap, ci)
 // with A gathered on the fly from the zero-padded input (no im2col buffer).
-// One wave computes a 16-pixel x 16-channel tile with
-// v_mfma_f32_16x16x16_bf16 (__builtin_amdgcn_mfma_f32_16x16x16bf16_1k):
-//   A fragment: lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
-//   B fragment: lane l holds B[4 (l / 16) + i][l % 16]
-//   C fragment: lane l holds C[4 (l / 16) + i][l % 16]
-// Cin is a multiple of 4, so the 4 consecutive k of a lane share one tap and
-// are one 8-byte load.  The input gradient (dgrad) is the same kernel on the
-// output gradient with the weights flipped and transposed; the weight
-// gradient (wgrad) reduces over pixels with the same instruction and writes
+// The forward convolution uses gfx950's v_mfma_f32_16x16x32_bf16
+// (__builtin_amdgcn_mfma_f32_16x16x32_bf16), lane layout checked on the GPU:
+//   A fragment: lane l holds A[l % 16][8 (l / 16) + i], i = 0..7
+//   B fragment: lane l holds B[8 (l / 16) + i][l % 16]
+//   C fragment: lane l holds C[4 (l / 16) + i][l % 16], i = 0..3
+// one wave computing 16 pixels x every output channel.  The input gradient
+// (dgrad) is the same kernel on the output gradient with the weights flipped
+// and transposed.  The weight gradient (wgrad) reduces over pixels with the
+// same instruction on operands staged transposed in LDS, and writes
 // per-chunk partial sums that a second kernel adds in a fixed order.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 
 namespace {
 
@@ -44,6 +45,9 @@ int ufail(const char* fmt, ...)
     } while (0)
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t b)
@@ -61,12 +65,33 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 // ---------------------------------------------------------------------------
 // implicit-GEMM convolution, KS = 3 (padding 1) or 1
-// x: [N][H][W][Cin] bf16; wt: [Cout][Kp] bf16 with Kp = roundup(KS*KS*Cin, 16),
+// x: [N][H][W][Cin] bf16; wt: [Cout][Kp] bf16 with Kp = roundup(KS*KS*Cin, 32),
 // k = tap * Cin + ci, tap = (dy + 1) * KS + (dx + 1); zero padded beyond K.
 // out: bf16 [M][Cout] (y16) and/or fp32 (y32); bias fp32 or null; relu 0/1.
-// grid: (ceil(M / 64), ceil(Cout / 16)); block 256 = 4 waves x 16 pixels.
+// One wave computes 16 pixels x all Cout channels (NCT tiles of 16): the A
+// fragment (the pixels' input patch, gathered once per K step) feeds NCT
+// v_mfma_f32_16x16x32_bf16, each with its own weight fragment.  A lane's 8
+// consecutive k are two groups of 4 (Cin is a multiple of 4, so a group never
+// straddles a tap): two 8-byte loads.  The (tap, ci) of a lane's first k
+// advance by 32 per step without divisions.
+// grid: (ceil(M / 64)); block 256 = 4 waves x 16 pixels.
 // ---------------------------------------------------------------------------
+// four consecutive input channels ci..ci+3 of one tap as raw bf16 bits
 template <int KS>
+__device__ __forceinline__ u32x2 patch4(const uint16_t* __restrict__ x, bool valid, int tap, int ci, int an,
+                                        int ah, int aw, int H, int W, int Cin)
+{
+    u32x2 v = {0u, 0u};
+    if (!valid || tap >= KS * KS) return v;
+    const int ty = (KS == 3) ? (tap >= 6 ? 1 : (tap >= 3 ? 0 : -1)) : 0;
+    const int tx = (KS == 3) ? tap - 3 * (ty + 1) - 1 : 0;
+    const int hh = ah + ty, ww = aw + tx;
+    if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+        v = *reinterpret_cast<const u32x2*>(x + (((long)an * H + hh) * W + ww) * Cin + ci);
+    return v;
+}
+
+template <int KS, int NCT>
 __global__ void __launch_bounds__(256)
 k_conv_igemm(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wt,
              const float* __restrict__ bias, uint16_t* __restrict__ y16, float* __restrict__ y32,
@@ -74,118 +99,216 @@ k_conv_igemm(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wt,
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const long M = (long)N * H * W;
-    const long m0 = ((long)blockIdx.x * 4 + wave) * 16;
-    const int n0 = blockIdx.y * 16;
+    const int M = N * H * W;
+    const int m0 = (blockIdx.x * 4 + wave) * 16;
+    if (m0 >= M) return;
     const int r = lane & 15;
     const int q = lane >> 4;
-    const long ma = m0 + r;                    // pixel of this lane's A row
-    int an = 0, ah = 0, aw = 0;
+    const int ma = m0 + r;                     // pixel of this lane's A row
     const bool avalid = ma < M;
-    if (avalid) {
-        aw = (int)(ma % W);
-        ah = (int)((ma / W) % H);
-        an = (int)(ma / ((long)W * H));
-    }
-    const int co_b = n0 + r;                   // output channel of this lane's B column
-    const int K = KS * KS * Cin;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < Kp; kb += 16) {
-        const int k = kb + 4 * q;
-        bf16x4 a = {0, 0, 0, 0};
-        if (avalid && k < K) {
-            const int tap = k / Cin;
-            const int ci = k - tap * Cin;
-            const int dy = (KS == 3) ? tap / 3 - 1 : 0;
-            const int dx = (KS == 3) ? tap % 3 - 1 : 0;
-            const int hh = ah + dy, ww = aw + dx;
-            if (hh >= 0 && hh < H && ww >= 0 && ww < W)
-                a = *reinterpret_cast<const bf16x4*>(x + (((long)an * H + hh) * W + ww) * Cin + ci);
-        }
-        bf16x4 b = {0, 0, 0, 0};
-        if (co_b < Cout) b = *reinterpret_cast<const bf16x4*>(wt + (long)co_b * Kp + k);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    const int co = n0 + r;
-    if (co >= Cout) return;
-    const float bv = bias ? bias[co] : 0.f;
+    const int aw = ma % W, ah = (ma / W) % H, an = ma / (W * H);
+    // (tap, ci) of k = kb + 8q, stepped with kb
+    int tap = (8 * q) / Cin;
+    int ci = 8 * q - tap * Cin;
+    f32x4 acc[NCT];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const long m = m0 + 4 * q + i;
-        if (m >= M) continue;
-        float v = acc[i] + bv;
-        if (relu) v = v > 0.f ? v : 0.f;
-        if (y16) y16[m * Cout + co] = f2bf(v);
-        if (y32) y32[m * Cout + co] = v;
+    for (int t = 0; t < NCT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < Kp; kb += 32) {
+        const bool second_next = ci + 4 >= Cin;
+        const u32x2 lo = patch4<KS>(x, avalid, tap, ci, an, ah, aw, H, W, Cin);
+        const u32x2 hi = patch4<KS>(x, avalid, second_next ? tap + 1 : tap, second_next ? ci + 4 - Cin : ci + 4,
+                                    an, ah, aw, H, W, Cin);
+        const u32x4 raw = {lo.x, lo.y, hi.x, hi.y};
+        const bf16x8 a = __builtin_bit_cast(bf16x8, raw);
+        const int k = kb + 8 * q;
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) {
+            const int co = 16 * t + r;
+            bf16x8 b = {};
+            if (co < Cout) b = *reinterpret_cast<const bf16x8*>(wt + (long)co * Kp + k);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+        ci += 32;
+        while (ci >= Cin) {
+            ci -= Cin;
+            ++tap;
+        }
     }
-}
-
-// ---------------------------------------------------------------------------
-// weight gradient: dW[co][k] = sum_m dY[m][co] A(m, k).
-// grid: (ceil(Kp / 64), ceil(Cout / 16), chunks); wave w handles k tile
-// blockIdx.x * 4 + w; pixels [chunk * per, (chunk + 1) * per).
-// part: [chunks][Cout][Kp] fp32 partial sums.
-// ---------------------------------------------------------------------------
-template <int KS>
-__global__ void __launch_bounds__(256)
-k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
-             float* __restrict__ part, int N, int H, int W, int Cin, int Cout, int Kp, long per)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int k0 = (blockIdx.x * 4 + wave) * 16;
-    if (k0 >= Kp) return;
-    const int co0 = blockIdx.y * 16;
-    const long M = (long)N * H * W;
-    const long p_begin = (long)blockIdx.z * per;
-    long p_end = p_begin + per;
-    if (p_end > M) p_end = M;
-    const int r = lane & 15, q = lane >> 4;
-    const int K = KS * KS * Cin;
-    // B column of this lane: k = k0 + r -> (tap, ci, dy, dx)
-    const int kb = k0 + r;
-    const bool kvalid = kb < K;
-    int ci = 0, dyy = 0, dxx = 0;
-    if (kvalid) {
-        const int tap = kb / Cin;
-        ci = kb - tap * Cin;
-        dyy = (KS == 3) ? tap / 3 - 1 : 0;
-        dxx = (KS == 3) ? tap % 3 - 1 : 0;
-    }
-    const int coa = co0 + r;                  // A row: output channel
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (long p0 = p_begin; p0 < p_end; p0 += 16) {
-        bf16x4 a, b;
+#pragma unroll
+    for (int t = 0; t < NCT; ++t) {
+        const int co = 16 * t + r;
+        if (co >= Cout) continue;
+        const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const long m = p0 + 4 * q + i;
-            short av = 0, bv = 0;
-            if (m < p_end) {
-                if (coa < Cout) av = (short)dy[m * Cout + coa];
-                if (kvalid) {
-                    const int w = (int)(m % W);
-                    const int h = (int)((m / W) % H);
-                    const long n = m / ((long)W * H);
-                    const int hh = h + dyy, ww = w + dxx;
-                    if (hh >= 0 && hh < H && ww >= 0 && ww < W)
-                        bv = (short)x[((n * H + hh) * W + ww) * Cin + ci];
-                }
-            }
-            a[i] = av;
-            b[i] = bv;
+            const int m = m0 + 4 * q + i;
+            if (m >= M) continue;
+            float v = acc[t][i] + bv;
+            if (relu) v = v > 0.f ? v : 0.f;
+            if (y16) y16[(long)m * Cout + co] = f2bf(v);
+            if (y32) y32[(long)m * Cout + co] = v;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    float* out = part + (long)blockIdx.z * Cout * Kp;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int co = co0 + 4 * q + i;
-        const int k = k0 + r;
-        if (co < Cout && k < Kp) out[(long)co * Kp + k] = acc[i];
     }
 }
 
-// sum partials over chunks (fixed order) into dw; bias gradient from dY
+// ---------------------------------------------------------------------------
+// weight gradient: dW[co][tap * Cin + ci] = sum_m dY[m][co] X(m shifted by tap)[ci]
+// A GEMM whose reduction runs over pixels, while both operands are stored
+// channel-contiguous per pixel, so each workgroup stages WG_PIX pixels of dY
+// and of the tap-shifted input in LDS transposed ([channel][pixel]); a
+// fragment of 8 consecutive pixels is then one 16-byte LDS read, and
+// v_mfma_f32_16x16x32_bf16 reduces 32 pixels per instruction.
+// grid: (KS * KS taps, chunks); block 256 = 4 waves; the Cout x Cin tile of
+// the tap is split into 16 x 16 MFMA tiles spread over the waves (at most
+// WG_TILES per wave).  part: [chunks][Cout][Kp] fp32 partial sums; the
+// workgroups of tap 0 also write the zero padding k >= KS*KS*Cin and the
+// chunk's bias partials: [chunks][Cout] after the weight partials.
+// ---------------------------------------------------------------------------
+constexpr int WG_TILES = 8;         // MFMA tiles per wave (Cout/16 * Cin/16 <= 32)
+
+// v[j] holds channels c..c+3 of pixel j (bf16 pairs in two words); writes
+// row c + i, pixels 0..3, for i = 0..3 (rows LD apart), 8 bytes each
+template <int LD>
+__device__ __forceinline__ void store_t4(uint16_t* dst, const u32x2 v[4])
+{
+    // channel (low or high half of word w) of pixels j and j + 1 in one word
+    auto lo2 = [](unsigned a, unsigned b) { return (a & 0xffffu) | (b << 16); };
+    auto hi2 = [](unsigned a, unsigned b) { return (a >> 16) | (b & 0xffff0000u); };
+    const u32x2 r0 = {lo2(v[0].x, v[1].x), lo2(v[2].x, v[3].x)};
+    const u32x2 r1 = {hi2(v[0].x, v[1].x), hi2(v[2].x, v[3].x)};
+    const u32x2 r2 = {lo2(v[0].y, v[1].y), lo2(v[2].y, v[3].y)};
+    const u32x2 r3 = {hi2(v[0].y, v[1].y), hi2(v[2].y, v[3].y)};
+    *reinterpret_cast<u32x2*>(dst) = r0;
+    *reinterpret_cast<u32x2*>(dst + LD) = r1;
+    *reinterpret_cast<u32x2*>(dst + 2 * LD) = r2;
+    *reinterpret_cast<u32x2*>(dst + 3 * LD) = r3;
+}
+
+template <int KS, int WG_PIX>
+__global__ void __launch_bounds__(256)
+k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+             float* __restrict__ part, int N, int H, int W, int Cin, int Cout, int Kp, int per)
+{
+    constexpr int WG_LD = WG_PIX + 8;   // LDS row (bf16): 16-byte aligned, rows offset in banks
+    // dynamic LDS: [16 cot][WG_LD] dY, [16 cit][WG_LD] input, [WG_PIX] source pixels
+    extern __shared__ __attribute__((aligned(16))) uint16_t u_smem[];
+    uint16_t* sD = u_smem;
+    uint16_t* sX = sD + 16 * ((Cout + 15) / 16) * WG_LD;
+    int* sSrc = reinterpret_cast<int*>(sX + 16 * ((Cin + 15) / 16) * WG_LD);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int tap = blockIdx.x;
+    const int ty = (KS == 3) ? tap / 3 - 1 : 0, tx = (KS == 3) ? tap % 3 - 1 : 0;
+    const int M = N * H * W;
+    const int p_begin = blockIdx.y * per;
+    int p_end = p_begin + per;
+    if (p_end > M) p_end = M;
+    const int cot = (Cout + 15) / 16, cit = (Cin + 15) / 16;
+    const int ntile = cot * cit;
+    f32x4 acc[WG_TILES];
+#pragma unroll
+    for (int j = 0; j < WG_TILES; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // tap 0 also sums dY over its pixels (the bias gradient): dY^T times a
+    // column of ones, channel tile `wave`
+    const bool bias_wave = tap == 0 && wave < cot;
+    f32x4 accb = {0.f, 0.f, 0.f, 0.f};
+    const u32x4 ones_raw = {0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};   // bf16 1.0
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, ones_raw);
+    const int c4o = Cout / 4, c4i = Cin / 4;
+    for (int p0 = p_begin; p0 < p_end; p0 += WG_PIX) {
+        __syncthreads();                      // the previous step's fragments are read
+        if (tid < WG_PIX) {
+            const int m = p0 + tid;
+            int src = -1;
+            if (m < p_end) {
+                const int w = m % W, h = (m / W) % H, n = m / (W * H);
+                const int hh = h + ty, ww = w + tx;
+                if (hh >= 0 && hh < H && ww >= 0 && ww < W) src = (n * H + hh) * W + ww;
+            }
+            sSrc[tid] = src;
+        }
+        // dY rows of the step stored transposed: an item is 4 pixels x 4
+        // channels (four 8-byte loads, four 8-byte LDS stores)
+        for (int it = tid; it < (WG_PIX / 4) * c4o; it += 256) {
+            const int pq = it / c4o, c = 4 * (it - pq * c4o);
+            u32x2 v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = p0 + 4 * pq + j;
+                v[j] = u32x2{0u, 0u};
+                if (m < p_end) v[j] = *reinterpret_cast<const u32x2*>(dy + (long)m * Cout + c);
+            }
+            store_t4<WG_LD>(sD + c * WG_LD + 4 * pq, v);
+        }
+        // padding rows of a partial channel tile hold zeros
+        for (int it = tid; it < (16 * cot - Cout) * WG_PIX; it += 256)
+            sD[(Cout + it / WG_PIX) * WG_LD + it % WG_PIX] = 0;
+        __syncthreads();                      // sSrc ready
+        for (int it = tid; it < (WG_PIX / 4) * c4i; it += 256) {
+            const int pq = it / c4i, c = 4 * (it - pq * c4i);
+            u32x2 v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int src = sSrc[4 * pq + j];
+                v[j] = u32x2{0u, 0u};
+                if (src >= 0) v[j] = *reinterpret_cast<const u32x2*>(x + (long)src * Cin + c);
+            }
+            store_t4<WG_LD>(sX + c * WG_LD + 4 * pq, v);
+        }
+        for (int it = tid; it < (16 * cit - Cin) * WG_PIX; it += 256)
+            sX[(Cin + it / WG_PIX) * WG_LD + it % WG_PIX] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < WG_PIX; kk += 32) {
+            if (bias_wave) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(sD + (16 * wave + r) * WG_LD + kk + 8 * q);
+                accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ones, accb, 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < WG_TILES; ++j) {
+                const int t = wave + 4 * j;
+                if (t < ntile) {
+                    const int ct = t / cit, ctile = t - ct * cit;
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(sD + (16 * ct + r) * WG_LD + kk + 8 * q);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(sX + (16 * ctile + r) * WG_LD + kk + 8 * q);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    float* out = part + (long)blockIdx.y * Cout * Kp;
+#pragma unroll
+    for (int j = 0; j < WG_TILES; ++j) {
+        const int t = wave + 4 * j;
+        if (t >= ntile) continue;
+        const int ct = t / cit, ctile = t - ct * cit;
+        const int ci = 16 * ctile + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = 16 * ct + 4 * q + i;
+            if (co < Cout && ci < Cin) out[(long)co * Kp + tap * Cin + ci] = acc[j][i];
+        }
+    }
+    if (bias_wave && r == 0) {
+        // every column of the product holds the channel sums: lane r = 0 writes
+        float* pb = part + (long)gridDim.y * Cout * Kp + (long)blockIdx.y * Cout;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = 16 * wave + 4 * q + i;
+            if (co < Cout) pb[co] = accb[i];
+        }
+    }
+    if (tap == 0) {
+        const int K = KS * KS * Cin;
+        for (int it = tid; it < Cout * (Kp - K); it += 256) {
+            const int co = it / (Kp - K);
+            out[(long)co * Kp + K + (it - co * (Kp - K))] = 0.f;
+        }
+    }
+}
+
+// sum partials over chunks (fixed order) into dw (and the bias partials into db)
 __global__ void __launch_bounds__(256)
 k_reduce_chunks(const float* __restrict__ part, float* __restrict__ dw, long n, int chunks)
 {
@@ -194,23 +317,6 @@ k_reduce_chunks(const float* __restrict__ part, float* __restrict__ dw, long n, 
     float s = 0.f;
     for (int c = 0; c < chunks; ++c) s += part[(long)c * n + i];
     dw[i] = s;
-}
-
-__global__ void __launch_bounds__(256)
-k_bias_grad(const uint16_t* __restrict__ dy, float* __restrict__ db, long M, int C)
-{
-    // one workgroup per channel; fixed-order tree sum
-    __shared__ float sh[256];
-    const int c = blockIdx.x;
-    float s = 0.f;
-    for (long m = threadIdx.x; m < M; m += 256) s += bf2f(dy[m * C + c]);
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[c] = sh[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -344,17 +450,32 @@ int jsv_conv_fwd(const void* x, const void* wt, const float* bias, void* y16, fl
                  int H, int W, int Cin, int Cout, int ksize, int Kp, int relu, void* stream)
 {
     if (Cin % 4 != 0) return ufail("Cin must be a multiple of 4 (got %d)", Cin);
-    if (Kp % 16 != 0 || Kp < ksize * ksize * Cin) return ufail("bad Kp %d", Kp);
+    if (Kp % 32 != 0 || Kp < ksize * ksize * Cin) return ufail("bad Kp %d (a multiple of 32 >= K)", Kp);
     if (ksize != 1 && ksize != 3) return ufail("ksize must be 1 or 3");
+    if (Cout < 1 || Cout > 128) return ufail("Cout must be in [1, 128] (got %d)", Cout);
     const long M = (long)N * H * W;
-    dim3 gr((unsigned)((M + 63) / 64), (unsigned)((Cout + 15) / 16));
+    if (M > (1L << 30)) return ufail("too many pixels (%ld)", M);
+    if ((((uintptr_t)wt) & 15) != 0 || (((uintptr_t)x) & 7) != 0) return ufail("misaligned conv operands");
+    const dim3 gr((unsigned)((M + 63) / 64));
     hipStream_t s = (hipStream_t)stream;
-    if (ksize == 3)
-        hipLaunchKernelGGL(k_conv_igemm<3>, gr, dim3(256), 0, s, (const uint16_t*)x, (const uint16_t*)wt,
-                           bias, (uint16_t*)y16, y32, N, H, W, Cin, Cout, Kp, relu);
-    else
-        hipLaunchKernelGGL(k_conv_igemm<1>, gr, dim3(256), 0, s, (const uint16_t*)x, (const uint16_t*)wt,
-                           bias, (uint16_t*)y16, y32, N, H, W, Cin, Cout, Kp, relu);
+    const int nct = (Cout + 15) / 16;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, gr, dim3(256), 0, s, (const uint16_t*)x, (const uint16_t*)wt, bias,
+                           (uint16_t*)y16, y32, N, H, W, Cin, Cout, Kp, relu);
+    };
+    auto pick = [&](auto ks) {
+        constexpr int KS = decltype(ks)::value;
+        switch (nct) {
+        case 1: go(k_conv_igemm<KS, 1>); break;
+        case 2: go(k_conv_igemm<KS, 2>); break;
+        case 3: go(k_conv_igemm<KS, 3>); break;
+        case 4: go(k_conv_igemm<KS, 4>); break;
+        case 5: case 6: go(k_conv_igemm<KS, 6>); break;
+        default: go(k_conv_igemm<KS, 8>); break;
+        }
+    };
+    if (ksize == 3) pick(std::integral_constant<int, 3>{});
+    else pick(std::integral_constant<int, 1>{});
     ULAUNCH();
     return 0;
 }
@@ -363,21 +484,33 @@ int jsv_conv_fwd(const void* x, const void* wt, const float* bias, void* y16, fl
 int jsv_conv_wgrad(const void* x, const void* dy, float* dw, float* db, float* part, int chunks,
                    int N, int H, int W, int Cin, int Cout, int ksize, int Kp, void* stream)
 {
-    if (Cin % 4 != 0 || Kp % 16 != 0 || chunks < 1) return ufail("bad wgrad arguments");
+    if (Cin % 4 != 0 || Kp % 32 != 0 || chunks < 1) return ufail("bad wgrad arguments");
+    if (ksize != 1 && ksize != 3) return ufail("ksize must be 1 or 3");
+    if (Cout < 1 || Cout > 64 || Cout % 4 != 0 || Cin > 128 || Kp < ksize * ksize * Cin)
+        return ufail("wgrad needs Cout in [4, 64] and Cin <= 128, multiples of 4 (got %d, %d)", Cout, Cin);
+    if (((Cout + 15) / 16) * ((Cin + 15) / 16) > 4 * WG_TILES) return ufail("wgrad tile count too large");
+    if ((((uintptr_t)x) & 7) != 0 || (((uintptr_t)dy) & 7) != 0) return ufail("misaligned wgrad operands");
     const long M = (long)N * H * W;
-    long per = (M + chunks - 1) / chunks;
-    per = (per + 15) / 16 * 16;
-    dim3 gr((unsigned)((Kp + 63) / 64), (unsigned)((Cout + 15) / 16), (unsigned)chunks);
+    if (M > (1L << 30)) return ufail("too many pixels (%ld)", M);
+    const long per = (M + chunks - 1) / chunks;
+    const dim3 gr((unsigned)(ksize * ksize), (unsigned)chunks);
     hipStream_t s = (hipStream_t)stream;
+    // LDS sized by the layer's channel tiles, so narrow layers fit more
+    // workgroups per CU (their steps are latency-bound)
+    constexpr int pix = 64;
+    const int rows = 16 * ((Cout + 15) / 16) + 16 * ((Cin + 15) / 16);
+    const size_t lds = (size_t)rows * (pix + 8) * 2 + (size_t)pix * 4;
     if (ksize == 3)
-        hipLaunchKernelGGL(k_conv_wgrad<3>, gr, dim3(256), 0, s, (const uint16_t*)x, (const uint16_t*)dy,
-                           part, N, H, W, Cin, Cout, Kp, per);
+        hipLaunchKernelGGL((k_conv_wgrad<3, pix>), gr, dim3(256), lds, s, (const uint16_t*)x, (const uint16_t*)dy,
+                           part, N, H, W, Cin, Cout, Kp, (int)per);
     else
-        hipLaunchKernelGGL(k_conv_wgrad<1>, gr, dim3(256), 0, s, (const uint16_t*)x, (const uint16_t*)dy,
-                           part, N, H, W, Cin, Cout, Kp, per);
+        hipLaunchKernelGGL((k_conv_wgrad<1, pix>), gr, dim3(256), lds, s, (const uint16_t*)x, (const uint16_t*)dy,
+                           part, N, H, W, Cin, Cout, Kp, (int)per);
     const long n = (long)Cout * Kp;
     hipLaunchKernelGGL(k_reduce_chunks, g1(n), dim3(256), 0, s, part, dw, n, chunks);
-    if (db) hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)Cout), dim3(256), 0, s, (const uint16_t*)dy, db, M, Cout);
+    if (db)
+        hipLaunchKernelGGL(k_reduce_chunks, g1(Cout), dim3(256), 0, s, part + (long)chunks * n, db, (long)Cout,
+                           chunks);
     ULAUNCH();
     return 0;
 }

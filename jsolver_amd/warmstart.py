@@ -38,7 +38,8 @@ LAYERS = [
 
 
 def _kp(k, cin):
-    return int(math.ceil(k * k * cin / 16.0) * 16)
+    """Weight row length: k*k*cin padded to the 32-deep K step of the MFMA."""
+    return int(math.ceil(k * k * cin / 32.0) * 32)
 
 
 def _sp():
@@ -143,7 +144,7 @@ class UNet:
             _chk(lib.jsv_relu_bwd(_p(dy), _p(y), dy.numel(), _sp()))
         M = N * H * W
         chunks = max(1, min(256, M // 4096))
-        part = torch.empty(chunks * L.cout * L.kp, dtype=torch.float32, device=self.dev)
+        part = torch.empty(chunks * L.cout * (L.kp + 1), dtype=torch.float32, device=self.dev)
         _chk(lib.jsv_conv_wgrad(_p(x), _p(dy), _p(L.gw), _p(L.gb), _p(part), chunks, N, H, W, L.cin,
                                 L.cout, L.k, L.kp, _sp()))
         if not need_dx:

@@ -22,7 +22,7 @@ import torch
 
 from .grid_1d import grid_1D
 from .solver_2d import solve_batched_2d
-from .warmstart import UNet, features
+from .warmstart import UNet, _chk, _lib_unet, _p, _sp, features
 
 
 def grf_batch(B, n, h, corr, gen, dev):
@@ -372,6 +372,62 @@ class CurveProbe:
         self.log(f"[curve] {json.dumps(pt)}")
 
 
+def time_convs(m, batch, dev, reps=10, log=print):
+    """Per-layer time and achieved TFLOP/s of the network's convolutions on a
+    (batch, m, m) input, with HIP events on the launch stream: forward,
+    input gradient (dgrad, the forward kernel on flipped weights) and weight
+    gradient (wgrad: the MFMA pixel reduction, the chunk sum and the bias
+    sum).  Flops: 2 * pixels * Cout * k^2 * Cin per pass."""
+    from .warmstart import LAYERS
+    net = UNet(seed=0, device=dev)
+    res = {"e1": m, "e2": m // 2, "e3": m // 4, "b1": m // 8, "b2": m // 8, "d3": m // 4, "d2": m // 2,
+           "d1": m, "ou": m}
+    rows = []
+    for (name, cin, cout, k, relu) in LAYERS:
+        L = net.layers[name]
+        H = W = res[name[:2]]
+        x = (torch.randn((batch, H, W, cin), device=dev) * 0.5).to(torch.bfloat16).contiguous()
+        dy = (torch.randn((batch, H, W, cout), device=dev) * 0.5).to(torch.bfloat16).contiguous()
+        flops = 2.0 * batch * H * W * cout * k * k * cin
+
+        def timed(fn):
+            fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / 1e3 / reps
+
+        t_f = timed(lambda: net._conv(L, x, batch, H, W))
+        lib = _lib_unet()
+        M = batch * H * W
+        chunks = max(1, min(256, M // 4096))
+        part = torch.empty(chunks * L.cout * (L.kp + 1), dtype=torch.float32, device=dev)
+
+        def wgrad():
+            _chk(lib.jsv_conv_wgrad(_p(x), _p(dy), _p(L.gw), _p(L.gb), _p(part), chunks, batch, H, W, L.cin,
+                                    L.cout, L.k, L.kp, _sp()))
+        t_w = timed(wgrad)
+        wd, kpd = L.dgrad_weight16()
+
+        def dgrad():
+            dx = torch.empty((batch, H, W, cin), dtype=torch.bfloat16, device=dev)
+            _chk(lib.jsv_conv_fwd(_p(dy), _p(wd), None, _p(dx), None, batch, H, W, cout, cin, k, kpd, 0, _sp()))
+        t_d = timed(dgrad)
+        row = {"layer": name, "hw": H, "cin": cin, "cout": cout, "k": k, "gflop": flops / 1e9,
+               "fwd_us": t_f * 1e6, "fwd_tflops": flops / t_f / 1e12,
+               "dgrad_us": t_d * 1e6, "dgrad_tflops": flops / t_d / 1e12,
+               "wgrad_us": t_w * 1e6, "wgrad_tflops": flops / t_w / 1e12}
+        rows.append(row)
+        log(f"[convs] {json.dumps(row)}")
+    tot = {kind: sum(r[f"{kind}_us"] for r in rows) for kind in ("fwd", "dgrad", "wgrad")}
+    gf = sum(r["gflop"] for r in rows)
+    summary = {kind: {"us": tot[kind], "tflops": gf * 1e9 / (tot[kind] * 1e-6) / 1e12} for kind in tot}
+    return {"m": m, "batch": batch, "layers": rows, "total": summary}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--samples", type=int, default=20000)
@@ -392,6 +448,8 @@ def main(argv=None):
     ap.add_argument("--curve-heldout", type=int, default=256, help="held-out samples for the curve's MSE")
     ap.add_argument("--diagnose", type=int, default=0,
                     help="problems at --eval-m for the gain diagnosis (0: none)")
+    ap.add_argument("--time-convs", action="store_true",
+                    help="time every convolution (forward, dgrad, wgrad) on a --batch x --m input and exit")
     ap.add_argument("--out", default=None)
     ap.add_argument("--eval-only", default=None, help="state file of a trained network: evaluate it only")
     a = ap.parse_args(argv)
@@ -409,6 +467,9 @@ def main(argv=None):
                                                          guess_sweeps=sw, criterion=criterion)
         return ev
 
+    if a.time_convs:
+        print(json.dumps(time_convs(a.m, a.batch, dev, log=log)), flush=True)
+        return
     if a.eval_only:
         net = UNet(seed=a.seed, device=dev)
         net.load_state_dict(torch.load(a.eval_only, weights_only=True))
