@@ -309,6 +309,23 @@ k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
 }
 
 // sum partials over chunks (fixed order) into dw (and the bias partials into db)
+// few outputs, many chunks: one workgroup per output, a fixed-order tree
+__global__ void __launch_bounds__(256)
+k_reduce_chunks_tree(const float* __restrict__ part, float* __restrict__ dw, long n, int chunks)
+{
+    __shared__ float sh[256];
+    const long i = blockIdx.x;
+    float v = 0.f;
+    for (int c = threadIdx.x; c < chunks; c += 256) v += part[(long)c * n + i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dw[i] = sh[0];
+}
+
 __global__ void __launch_bounds__(256)
 k_reduce_chunks(const float* __restrict__ part, float* __restrict__ dw, long n, int chunks)
 {
@@ -507,10 +524,15 @@ int jsv_conv_wgrad(const void* x, const void* dy, float* dw, float* db, float* p
         hipLaunchKernelGGL((k_conv_wgrad<1, pix>), gr, dim3(256), lds, s, (const uint16_t*)x, (const uint16_t*)dy,
                            part, N, H, W, Cin, Cout, Kp, (int)per);
     const long n = (long)Cout * Kp;
-    hipLaunchKernelGGL(k_reduce_chunks, g1(n), dim3(256), 0, s, part, dw, n, chunks);
-    if (db)
-        hipLaunchKernelGGL(k_reduce_chunks, g1(Cout), dim3(256), 0, s, part + (long)chunks * n, db, (long)Cout,
-                           chunks);
+    // few outputs: a workgroup each (the chunk loop would be one long serial chain)
+    auto reduce = [&](const float* src, float* dst, long cnt) {
+        if (cnt < 16384)
+            hipLaunchKernelGGL(k_reduce_chunks_tree, dim3((unsigned)cnt), dim3(256), 0, s, src, dst, cnt, chunks);
+        else
+            hipLaunchKernelGGL(k_reduce_chunks, g1(cnt), dim3(256), 0, s, src, dst, cnt, chunks);
+    };
+    reduce(part, dw, n);
+    if (db) reduce(part + (long)chunks * n, db, (long)Cout);
     ULAUNCH();
     return 0;
 }
