@@ -200,6 +200,22 @@ def main():
     torch.cuda.synchronize()
     t64a = time.perf_counter() - t0
     log(f"fp64 (adaptive checks): {n64a} cycles, {t64a * 1e3:.1f} ms")
+    # FP64 with the per-cycle test read by the host after every cycle (the
+    # device-resident test switched off), for comparison
+    from jsolver_amd import _lib, _solver
+    plan64 = _solver.get_plan(3, [m, m, m], [g.idel for g in grids], "fp64")
+    lib = _lib.load()
+    _lib.check(lib.jsv_plan_set_tuning(plan64.h, _lib.JSV_TUNE_DEVICE_STOP, 0))
+    try:
+        run("fp64")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, n64h, info64h = run("fp64")
+        torch.cuda.synchronize()
+        t64h = time.perf_counter() - t0
+    finally:
+        _lib.check(lib.jsv_plan_set_tuning(plan64.h, _lib.JSV_TUNE_DEVICE_STOP, 1))
+    log(f"fp64 (host reads every test): {n64h} cycles, {t64h * 1e3:.1f} ms")
 
     # ---- mixed precision: the timed steps ----------------------------------
     for _ in range(a.warmup):
@@ -262,7 +278,12 @@ def main():
                                f"dynamic to rel. residual {a.rtol:g}",
                    "unknowns": unknowns, "cycle": a.cycle, "inner_cycles": a.inner_cycles},
         "fp64": {"time_to_solution_s": round(t64, 5), "cycles": n64,
-                 "rel_residual": info64["rel_residual"], "residual_checked": "every cycle"},
+                 "rel_residual": info64["rel_residual"],
+                 "residual_checked": "every cycle, on the device (read back once per block of cycles)",
+                 "time_vs_adaptive_checks": round(t64 / t64a, 4)},
+        "fp64_host_checks": {"time_to_solution_s": round(t64h, 5), "cycles": n64h,
+                             "rel_residual": info64h["rel_residual"],
+                             "residual_checked": "every cycle, read by the host after each cycle"},
         "fp64_adaptive_checks": {"time_to_solution_s": round(t64a, 5), "cycles": n64a,
                                  "rel_residual": info64a["rel_residual"],
                                  "speedup_mixed_vs_this": round(t64a / tm, 3)},

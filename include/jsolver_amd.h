@@ -140,7 +140,16 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         runs the first half of its first sweep (its phi is
  *                         zero then), and the first half of the
  *                         post-smoothing applies the prolongation on the fly;
- *                         0: separate launches */
+ *                         0: separate launches
+ *   JSV_TUNE_DEVICE_STOP  1 (default): an FP64 dynamic solve of one problem
+ *                         that tests every cycle (JSV_STOP_DISTANCE, or
+ *                         JSV_STOP_REL_RESIDUAL with check_every = 1) runs the
+ *                         test on the device inside each cycle and reads it
+ *                         back once per block of cycles; 0: the host reads
+ *                         every test.  Count and phi are the same either way;
+ *                         a relative residual summed inside the first sweep
+ *                         of a cycle may differ from the host's sum in the
+ *                         last bits */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -156,6 +165,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_COARSE_SOLO 12
 #define JSV_TUNE_COARSE_THREADS 13
 #define JSV_TUNE_FUSE_HALVES 14
+#define JSV_TUNE_DEVICE_STOP 15
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

@@ -147,11 +147,18 @@ struct jsv_plan {
     std::vector<CoarseProg> progs;
     hipStream_t cap_stream = nullptr;   // private stream for graph capture
     struct Graph {
-        int prec, gamma, fcycle, skip_pre, nu1, nu2, use_act;
+        int prec, gamma, fcycle, skip_pre, nu1, nu2, use_act, stop_mode;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
     bool use_act = false;           // kernels consult d_act (dynamic batched solves)
+    // stop rule on the device (dynamic solves of one problem that test every cycle)
+    int device_stop = 1;            // tuning: use it where it applies (0: the host reads every test)
+    int stop_mode = 0;              // cycles being built: 0 none, 1 relative residual, 2 distance
+    jsv::StopCtl* d_stop = nullptr; // control block
+    jsv::StopCtl* h_stop = nullptr; // pinned mirror
+    double* d_hist = nullptr;       // stop values (StopCtl::hist)
+    long hist_cap = 0;
     std::vector<void*> allocs;
     int64_t bytes = 0;
     const int* act_ptr() const { return use_act ? d_act : nullptr; }
@@ -249,7 +256,7 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
     if (lds > 160 * 1024) return fail("fused sweep tile needs %zu bytes of LDS", lds);
     hipLaunchKernelGGL((jsv::k_rbsweep3<T, TY, NT>), dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds,
                        s, H.phi[l], H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk,
-                       ytiles);
+                       ytiles, P->act_ptr());
     return 0;
 }
 
@@ -259,11 +266,12 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
 // ring row width of the pair kernel instantiated for level g (its NHMAX)
 int pair_nhm(const Geo& g) { return (g.nx + 1) / 2 <= 130 ? 130 : 258; }
 
-size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1)
+size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1, bool res = false)
 {
     const int nhm = pair_nhm(g);
     size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * nhm + (size_t)8 * np * 1024;
     if (gc != nullptr) e += (size_t)3 * (ty / 2 + 4) * (nhm + 1);
+    if (res) e += (size_t)3 * (ty + 2) * nhm + (size_t)np * 1024;   // ring of phi_in's red values, held black values
     return w * e;
 }
 
@@ -306,8 +314,12 @@ int pair_sweep_per_cu(const Geo& g, int ty)
     }
 }
 
+// res (FP64, no prolongation): also sum the squared residual of phi_in into
+// one partial per workgroup at res, their count to *nres (0: this shape has
+// no such variant, the sweep ran without it)
 template <typename T, int TY, int NHM, int NP>
-int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s)
+int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s,
+                       double* res, long* nres)
 {
     const Geo& g = H.g[l];
     constexpr int NT = 1024;
@@ -329,10 +341,22 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
     static const bool wr_pro = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP, 1, true>);
     static const bool wr_plain = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true>);
     const bool wr = ((g.mx / 2) / NP) % 64 == 0 && (pc != nullptr ? wr_pro : wr_plain);
-    auto go = [&](auto kern, const T* pcv) {
-        hipLaunchKernelGGL(kern, grid, dim3(NT), lds, s, H.phi[l], H.phi2[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l],
-                           H.cc[l], g, zchunk, ytiles, pcv, gc);
+    auto go = [&](auto kern, const T* pcv, size_t lds_k = 0, double* rp = nullptr) {
+        hipLaunchKernelGGL(kern, grid, dim3(NT), lds_k > 0 ? lds_k : lds, s, H.phi[l], H.phi2[l], H.rhs[l],
+                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp);
     };
+    if (nres != nullptr) *nres = 0;
+    if constexpr (sizeof(T) == 8 && NP == 1 && (TY == 5 || TY == 4 || TY == 2)) {
+        const size_t lds_r = pair_sweep_lds(g, TY, sizeof(T), nullptr, NP, true);
+        if (res != nullptr && pc == nullptr && lds_r <= 160 * 1024 && (long)grid.x <= P->npartials) {
+            static const bool wr_res = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true, true>);
+            const bool wrr = ((g.mx / 2) / NP) % 64 == 0 && wr_res;
+            wrr ? go(jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true, true>, (const T*)nullptr, lds_r, res)
+                : go(jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, false, true>, (const T*)nullptr, lds_r, res);
+            *nres = (long)grid.x;
+            return 0;
+        }
+    }
     if (pc != nullptr)
         wr ? go(jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP, 1, true>, pc)
            : go(jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP, 1, false>, pc);
@@ -343,21 +367,23 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
 }
 
 template <typename T, int TY, int NP>
-int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s)
+int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s, double* res,
+                       long* nres)
 {
-    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130, NP>(P, H, l, zchunk, pc, s);
-    return launch_rbsweep3p_t<T, TY, 258, NP>(P, H, l, zchunk, pc, s);
+    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130, NP>(P, H, l, zchunk, pc, s, res, nres);
+    return launch_rbsweep3p_t<T, TY, 258, NP>(P, H, l, zchunk, pc, s, res, nres);
 }
 
 template <typename T, int NP>
-int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zchunk, const T* pc, hipStream_t s)
+int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zchunk, const T* pc, hipStream_t s,
+                          double* res, long* nres)
 {
     switch (rows) {
-    case 14: return launch_rbsweep3p_t<T, 14, NP>(P, H, l, zchunk, pc, s);
-    case 6: return launch_rbsweep3p_t<T, 6, NP>(P, H, l, zchunk, pc, s);
-    case 5: return launch_rbsweep3p_t<T, 5, NP>(P, H, l, zchunk, pc, s);
-    case 4: return launch_rbsweep3p_t<T, 4, NP>(P, H, l, zchunk, pc, s);
-    default: return launch_rbsweep3p_t<T, 2, NP>(P, H, l, zchunk, pc, s);
+    case 14: return launch_rbsweep3p_t<T, 14, NP>(P, H, l, zchunk, pc, s, res, nres);
+    case 6: return launch_rbsweep3p_t<T, 6, NP>(P, H, l, zchunk, pc, s, res, nres);
+    case 5: return launch_rbsweep3p_t<T, 5, NP>(P, H, l, zchunk, pc, s, res, nres);
+    case 4: return launch_rbsweep3p_t<T, 4, NP>(P, H, l, zchunk, pc, s, res, nres);
+    default: return launch_rbsweep3p_t<T, 2, NP>(P, H, l, zchunk, pc, s, res, nres);
     }
 }
 
@@ -368,7 +394,8 @@ int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zc
 // times the share of pair rows that are not halo, TY / (TY + 2); the z-chunk
 // then gives each slot one tile.
 template <typename T>
-int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s)
+int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s, double* res = nullptr,
+                     long* nres = nullptr)
 {
     const Geo& g = H.g[l];
     const Geo* gc = pc != nullptr ? &H.g[l + 1] : nullptr;
@@ -412,16 +439,19 @@ int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStrea
         }
         zchunk = (g.mz - 1 + zchunks - 1) / zchunks;
     }
-    if (np == 2) return launch_rbsweep3p_rows<T, 2>(P, H, l, rows, zchunk, pc, s);
-    return launch_rbsweep3p_rows<T, 1>(P, H, l, rows, zchunk, pc, s);
+    if (np == 2) return launch_rbsweep3p_rows<T, 2>(P, H, l, rows, zchunk, pc, s, res, nres);
+    return launch_rbsweep3p_rows<T, 1>(P, H, l, rows, zchunk, pc, s, res, nres);
 }
 
 // tile rows x threads per workgroup of the fused sweep (tunable; default by precision)
 template <typename T>
-int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr)
+int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr,
+                    double* res = nullptr, long* nres = nullptr)
 {
+    if (nres != nullptr) *nres = 0;
     if (pc != nullptr) return launch_rbsweep3p<T>(P, H, l, pc, s);
-    if (P->sweep_variant == 2 && pair_sweep_fits(H.g[l], sizeof(T))) return launch_rbsweep3p<T>(P, H, l, nullptr, s);
+    if (P->sweep_variant == 2 && pair_sweep_fits(H.g[l], sizeof(T)))
+        return launch_rbsweep3p<T>(P, H, l, nullptr, s, res, nres);
     int rows = P->sweep_rows > 0 ? P->sweep_rows : (sizeof(T) == 4 ? 16 : 4);
     const int nt = P->sweep_threads > 0 ? P->sweep_threads : 1024;
     if (nt == 256) {
@@ -465,10 +495,14 @@ bool halves_path(const jsv_plan* P, const Hier<T>& H, int l)
 // see prolong_fusable; or, with fuse_halves, the first half of the halves path).
 // red_done: the first half already ran inside the restriction that produced
 // this level (fuse_halves; halves path only), so only the second half runs.
+// res / nres: see launch_rbsweep3p_t (fused out-of-place sweeps only; *nres
+// is 0 where the residual was not summed)
 template <typename T>
-int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr, bool red_done = false)
+int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr, bool red_done = false,
+           double* res = nullptr, long* nres = nullptr)
 {
     const Geo& g = H.g[l];
+    if (nres != nullptr) *nres = 0;
     if (P->dim == 2) {
         if (P->offdiag) {
             // the reference's sub-colour order (0,0),(1,1),(0,1),(1,0)
@@ -495,7 +529,7 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
         }
     } else if (P->fused_min_m > 0 && g.mz >= P->fused_min_m && H.phi2[l] != nullptr) {
         // fused out-of-place sweep, then swap the level's phi buffers
-        RETCHK(launch_rbsweep3<T>(P, H, l, s, pc));
+        RETCHK(launch_rbsweep3<T>(P, H, l, s, pc, res, nres));
         std::swap(H.phi[l], H.phi2[l]);
     } else {
         // Slab schedule: red on z-slab k, then black on slab k-1, so the black
@@ -516,15 +550,15 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
             // halves path with a fused first half
             if (pc != nullptr)
                 hipLaunchKernelGGL(jsv::k_rb3_pro<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
-                                   H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, 1, pc, H.g[l + 1]);
+                                   H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, 1, pc, H.g[l + 1], P->act_ptr());
             hipLaunchKernelGGL(jsv::k_rb3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l], H.cx[l],
-                               H.cy[l], H.cz[l], H.cc[l], g, 0, 2);
+                               H.cy[l], H.cz[l], H.cc[l], g, 0, 2, P->act_ptr());
             return 0;
         }
         auto launch = [&](int par, int za, int zb) {
             dim3 gs(gr.x, gr.y, (unsigned)(zb - za));
             hipLaunchKernelGGL(jsv::k_rb3<T>, gs, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l], H.rhs[l],
-                               H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, par, za);
+                               H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, par, za, P->act_ptr());
         };
         const int zend = g.mz + 1;
         int prev_a = -1, prev_b = -1;
@@ -564,18 +598,18 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
         } else {
             if (gf.n < (1L << 24)) {
                 hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
-                                   H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 1);
+                                   H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 1, P->act_ptr());
             } else {
                 constexpr int RB = 16;
                 const dim3 gt(gf_half.x, (unsigned)((gf.my - 1 + RB - 1) / RB), gf_half.z);
                 hipLaunchKernelGGL((jsv::k_resid_first3<T, RB>), gt, dim3(jsv::BX, RB), 0, s, H.res[l],
-                                   H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 0);
+                                   H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 0, P->act_ptr());
             }
             // the gather indexes the compact residuals in 32 bits
             if ((long)(gf.mz - 1) * (gf.my - 1) * (gf.mx / 2) >= (1L << 32))
                 return fail("level %d too large for the two-pass restriction", l);
             hipLaunchKernelGGL(jsv::k_restrict_gather3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
-                               H.rhs[l + 1], H.res[l], gf, gc, zphi, rz);
+                               H.rhs[l + 1], H.res[l], gf, gc, zphi, rz, P->act_ptr());
         }
         return 0;
     }
@@ -586,7 +620,8 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
                            P->act_ptr(), zphi, rz);
     else
         hipLaunchKernelGGL(jsv::k_restrict3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.rhs[l + 1],
-                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, gc, zphi, rz);
+                           H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, gc, zphi, rz,
+                           P->act_ptr());
     return 0;
 }
 
@@ -602,7 +637,7 @@ int prolong(const jsv_plan* P, Hier<T>& H, int l, int nu2, hipStream_t s)
                            H.phi[l + 1], gf, gc, full, P->act_ptr());
     } else {
         hipLaunchKernelGGL(jsv::k_prolong3<T>, gr, dim3(jsv::BX, jsv::BY), 0, s, H.phi[l],
-                           H.phi[l + 1], gf, gc, nu2 <= 0 ? 1 : 0);
+                           H.phi[l + 1], gf, gc, nu2 <= 0 ? 1 : 0, P->act_ptr());
     }
     return 0;
 }
@@ -936,19 +971,61 @@ bool image_current(const jsv_plan* P)
     return !P->img_key[pi].empty() && P->img_key[pi].back() == (const void*)P->op_version;
 }
 
-// multigrid_cycle (solver_2d.py:582-614 / solver_3d.py:588-620)
-// red_done: the restriction into this level already ran the first half of
-// its first sweep (fuse_halves)
 template <typename T>
 int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcyc, bool skip_pre,
-             hipStream_t s, bool red_done = false)
+             hipStream_t s, bool red_done = false);
+
+// The device stop test of the relative residual (StopCtl mode 0) for the
+// iterate `phi` a cycle starts from: nres partial sums of r^2 already in
+// P->partials (summed inside the first sweep), or, with nres = 0, a residual
+// pass over phi first.
+template <typename T>
+int stop_test_relres(const jsv_plan* P, Hier<T>& H, const T* phi, long nres, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 8) {
+        const Geo& g = H.g[0];
+        if (nres == 0) {
+            dim3 gr = grid_points(g, P->dim, g.mx - 1);
+            hipLaunchKernelGGL((jsv::k_residual<double, double, double>), gr, dim3(jsv::BX, jsv::BY), 0, s, phi,
+                               H.rhs[0], H.cx[0], H.cy[0], P->dim == 3 ? H.cz[0] : nullptr, H.cc[0],
+                               P->offdiag ? H.drr[0] : nullptr, P->offdiag ? H.dlr[0] : nullptr, g, (double*)nullptr,
+                               P->partials, P->dim, (const int*)P->d_act);
+            nres = (long)gr.x * gr.y * gr.z;
+        }
+        hipLaunchKernelGGL(jsv::k_stop_check, dim3(1), dim3(1024), 0, s, P->partials, nres, 0, P->d_stop, P->d_act);
+        return 0;
+    }
+    return fail("internal: the device stop test runs on FP64 cycles");
+}
+
+// multigrid_cycle (solver_2d.py:582-614 / solver_3d.py:588-620)
+// red_done: the restriction into this level already ran the first half of
+// its first sweep (fuse_halves).  test_first: level 0's first sweep is out of
+// place; it sums the residual of its input where it can, and the device stop
+// test runs right after it.
+template <typename T>
+int mg_cycle_body(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcyc, bool skip_pre,
+                  hipStream_t s, bool red_done, bool test_first)
+{
+    // the first sweep of the cycle, with the stop test behind it when asked
+    auto first_sweep = [&]() -> int {
+        const T* in = H.phi[l];
+        long nres = 0;
+        RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done, P->partials, &nres));
+        return stop_test_relres(P, H, in, nres, s);
+    };
     if (l == P->levels - 2) {
-        for (int i = 0; i < c.nu1 + c.nu2; ++i) RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0));
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) {
+            if (i == 0 && test_first) RETCHK(first_sweep());
+            else RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0));
+        }
         return 0;
     }
     if (!skip_pre)
-        for (int i = 0; i < c.nu1; ++i) RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0));
+        for (int i = 0; i < c.nu1; ++i) {
+            if (i == 0 && test_first) RETCHK(first_sweep());
+            else RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0));
+        }
     const int start = coarse_start(P, H);
     const jsv_plan::CoarseProg* pr = (start == l + 1) ? find_prog<T>(P, c, fcyc, start) : nullptr;
     if (pr != nullptr && pr->ops == nullptr) pr = nullptr;
@@ -1007,12 +1084,44 @@ int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcy
     return 0;
 }
 
+// One cycle from level l.  The top level of an FP64 solve whose stop rule runs
+// on the device (P->stop_mode) also carries the test (k_stop_check): for the
+// relative residual, of the iterate the cycle starts from -- summed inside
+// the cycle's first sweep when that sweep is out of place (its input stays
+// intact), else by a residual pass before the cycle; for the distance, of
+// the cycle's move, from a copy of its input.
+template <typename T>
+int mg_cycle(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, bool fcyc, bool skip_pre,
+             hipStream_t s, bool red_done)
+{
+    if (sizeof(T) != 8 || l != 0 || P->stop_mode == 0)
+        return mg_cycle_body(P, H, c, l, fcyc, skip_pre, s, red_done, false);
+    const Geo& g = H.g[0];
+    if (P->stop_mode == 2) {
+        const long n = g.n;
+        const unsigned nb = (unsigned)std::min<long>(std::min<long>((n + 255) / 256, 4L * P->n_cu), P->npartials);
+        hipLaunchKernelGGL(jsv::k_stop_copy, dim3(nb), dim3(256), 0, s, (double*)P->phi_old, (const double*)H.phi[0],
+                           n, (const jsv::StopCtl*)P->d_stop, (const int*)P->d_act);
+        RETCHK(mg_cycle_body(P, H, c, 0, fcyc, skip_pre, s, red_done, false));
+        hipLaunchKernelGGL(jsv::k_stop_maxdiff, dim3(nb), dim3(256), 0, s, (const double*)P->phi_old,
+                           (const double*)H.phi[0], n, P->partials, (const int*)P->d_act);
+        hipLaunchKernelGGL(jsv::k_stop_check, dim3(1), dim3(1024), 0, s, P->partials, (long)nb, 1, P->d_stop,
+                           P->d_act);
+        return 0;
+    }
+    const bool sweeps_first = (P->levels == 2) ? c.nu1 + c.nu2 >= 1 : (!skip_pre && c.nu1 >= 1);
+    if (sweeps_first && uses_fused(P, H, 0)) return mg_cycle_body(P, H, c, 0, fcyc, skip_pre, s, red_done, true);
+    RETCHK(stop_test_relres(P, H, H.phi[0], 0, s));
+    return mg_cycle_body(P, H, c, 0, fcyc, skip_pre, s, red_done, false);
+}
+
 void adj_drop_graphs(const jsv_plan* P);
 void adj_forget(const jsv_plan* P);
 
 void drop_graphs(jsv_plan* P)
 {
-    for (auto& gr : P->graphs) (void)hipGraphExecDestroy(gr.exec);
+    for (auto& gr : P->graphs)
+        if (gr.exec != nullptr) (void)hipGraphExecDestroy(gr.exec);   // nullptr: replayed eagerly
     P->graphs.clear();
     adj_drop_graphs(P);
 }
@@ -1034,7 +1143,8 @@ int run_cycle(jsv_plan* P, Hier<T>& H, const jsv_config& c, bool skip_pre, hipSt
     const int ua = P->use_act ? 1 : 0;
     for (auto& gr : P->graphs)
         if (gr.prec == prec && gr.gamma == c.gamma && gr.fcycle == (c.fcycle != 0) &&
-            gr.skip_pre == (skip_pre ? 1 : 0) && gr.nu1 == c.nu1 && gr.nu2 == c.nu2 && gr.use_act == ua) {
+            gr.skip_pre == (skip_pre ? 1 : 0) && gr.nu1 == c.nu1 && gr.nu2 == c.nu2 && gr.use_act == ua &&
+            gr.stop_mode == P->stop_mode) {
             if (gr.exec == nullptr) return mg_cycle(P, H, c, 0, c.fcycle != 0, skip_pre, s);
             HIPCHK(hipGraphLaunch(gr.exec, s));
             return 0;
@@ -1046,7 +1156,9 @@ int run_cycle(jsv_plan* P, Hier<T>& H, const jsv_config& c, bool skip_pre, hipSt
     HIPCHK(hipEventRecord(ev, s));
     HIPCHK(hipStreamWaitEvent(P->cap_stream, ev, 0));
     (void)hipEventDestroy(ev);
-    std::vector<T*> before = H.phi;
+    // the capture runs the host side of the cycle: out-of-place sweeps swap
+    // phi and phi2 of their level, so both lists are restored after it
+    std::vector<T*> before = H.phi, before2 = H.phi2;
     HIPCHK(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
     int rc = mg_cycle(P, H, c, 0, c.fcycle != 0, skip_pre, P->cap_stream);
     hipGraph_t graph = nullptr;
@@ -1054,10 +1166,16 @@ int run_cycle(jsv_plan* P, Hier<T>& H, const jsv_config& c, bool skip_pre, hipSt
     if (rc != 0) {
         if (graph) (void)hipGraphDestroy(graph);
         H.phi = before;
+        H.phi2 = before2;
         return rc;
     }
-    if (e != hipSuccess) return fail("graph capture failed: %s", hipGetErrorString(e));
-    jsv_plan::Graph gr{prec, c.gamma, c.fcycle != 0 ? 1 : 0, skip_pre ? 1 : 0, c.nu1, c.nu2, ua, nullptr};
+    if (e != hipSuccess) {
+        H.phi = before;
+        H.phi2 = before2;
+        return fail("graph capture failed: %s", hipGetErrorString(e));
+    }
+    jsv_plan::Graph gr{prec, c.gamma, c.fcycle != 0 ? 1 : 0, skip_pre ? 1 : 0, c.nu1, c.nu2, ua, P->stop_mode,
+                       nullptr};
     const bool same = (H.phi == before);
     if (same) {
         e = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
@@ -1066,6 +1184,7 @@ int run_cycle(jsv_plan* P, Hier<T>& H, const jsv_config& c, bool skip_pre, hipSt
     } else {
         (void)hipGraphDestroy(graph);
         H.phi = before;                          // replay eagerly instead
+        H.phi2 = before2;
     }
     P->graphs.push_back(gr);
     if (gr.exec != nullptr) {
@@ -1113,7 +1232,7 @@ int residual64(jsv_plan* P, const double* phi, const double* rhs, TO* rout, int 
     hipLaunchKernelGGL((jsv::k_residual<double, double, TO>), gr, dim3(jsv::BX, jsv::BY), 0, s,
                        phi, rhs, H.cx[0], H.cy[0], P->dim == 3 ? H.cz[0] : nullptr, H.cc[0],
                        P->offdiag ? H.drr[0] : nullptr, P->offdiag ? H.dlr[0] : nullptr, g, rout,
-                       P->partials, P->dim);
+                       P->partials, P->dim, (const int*)nullptr);
     const long per = (long)gr.x * gr.y * gr.z / nseg(P);
     hipLaunchKernelGGL(jsv::k_sum_partials, dim3((unsigned)nseg(P)), dim3(1024), 0, s, P->partials,
                        per, P->dscal + (long)slot * P->batch);
@@ -1236,8 +1355,10 @@ int jsv_plan_create_batched(int dim, int mx, int my, int mz, int batch, double i
         if (e == hipSuccess) e = hipHostMalloc((void**)&P->h_act, sizeof(int) * batch, hipHostMallocDefault);
         if (e == hipSuccess)
             e = hipHostMalloc((void**)&P->h_actk, sizeof(int) * ACTK_ROWS * batch, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&P->h_stop, sizeof(jsv::StopCtl), hipHostMallocDefault);
         if (e != hipSuccess) rc = fail("hipHostMalloc failed: %s", hipGetErrorString(e));
     }
+    if (rc == 0) rc = dalloc(P, (void**)&P->d_stop, sizeof(jsv::StopCtl));
     if (rc != 0) {
         std::string keep = g_err;
         jsv_plan_destroy(P);
@@ -1258,12 +1379,14 @@ int jsv_plan_destroy(jsv_plan* P)
 {
     if (P == nullptr) return 0;
     adj_forget(P);   // the adjoint state holds this plan's buffers
-    for (auto& gr : P->graphs) (void)hipGraphExecDestroy(gr.exec);
+    for (auto& gr : P->graphs)
+        if (gr.exec != nullptr) (void)hipGraphExecDestroy(gr.exec);   // nullptr: replayed eagerly
     if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
     for (void* p : P->allocs) (void)hipFree(p);
     if (P->hscal) (void)hipHostFree(P->hscal);
     if (P->h_act) (void)hipHostFree(P->h_act);
     if (P->h_actk) (void)hipHostFree(P->h_actk);
+    if (P->h_stop) (void)hipHostFree(P->h_stop);
     delete P;
     return 0;
 }
@@ -1357,6 +1480,10 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     }
     if (key == JSV_TUNE_FUSE_HALVES) {
         P->fuse_halves = value != 0 ? 1 : 0;
+        return 0;
+    }
+    if (key == JSV_TUNE_DEVICE_STOP) {
+        P->device_stop = value != 0 ? 1 : 0;
         return 0;
     }
     if (key == JSV_TUNE_FUSE_PROLONG) {
@@ -1878,6 +2005,94 @@ static void fill_result(const Track& T, jsv_result* res, int64_t* counts, double
     res->rel_residual = T.rel[0];
 }
 
+// A dynamic solve of one problem that tests every cycle (the reference's
+// distance rule, or the relative residual with check_every = 1) runs its test
+// on the device (k_stop_check inside each cycle): the host queues cycles in
+// blocks and reads the control block once per block; cycles queued behind a
+// stop do nothing.  The block length comes from the stop values' measured
+// contraction, so the last block ends near the stop.  Count, stop values and
+// phi are those of the test after every cycle.
+static bool device_stop_applies(const jsv_plan* P, const jsv_config* c, bool relres, int B)
+{
+    return P->device_stop != 0 && B == 1 && !c->fixed && c->max_iters > 0 && (!relres || c->check_every == 1);
+}
+
+static int solve_device_stop(jsv_plan* P, const jsv_config& c, bool relres, double rhs_norm,
+                             const double* first_old, Track& T, hipStream_t s)
+{
+    Hier<double>& H = P->h64;
+    const long n = H.g[0].n;
+    jsv::StopCtl& h = *P->h_stop;
+    long hl = 0;      // stop values kept on the device for the history
+    if (c.history != nullptr && c.history_len > 0) {
+        hl = (long)std::min<int64_t>(c.history_len, c.max_iters);
+        if (hl > P->hist_cap) {
+            HIPCHK(hipStreamSynchronize(s));
+            RETCHK(dfree(P, P->d_hist, sizeof(double) * (size_t)P->hist_cap));
+            P->d_hist = nullptr;
+            P->hist_cap = 0;
+            RETCHK(dalloc(P, (void**)&P->d_hist, sizeof(double) * (size_t)hl));
+            P->hist_cap = hl;
+        }
+    }
+    h = jsv::StopCtl{relres ? c.rtol : c.epsilon, rhs_norm, INFINITY, (long long)c.max_iters, 0, 0, 0, 0,
+                     hl > 0 ? P->d_hist : nullptr, (long long)hl};
+    HIPCHK(hipMemcpyAsync(P->d_stop, P->h_stop, sizeof(jsv::StopCtl), hipMemcpyHostToDevice, s));
+    // the distance of the first cycle is from the caller's phi
+    if (!relres && first_old != P->phi_old)
+        HIPCHK(hipMemcpyAsync(P->phi_old, first_old, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    P->use_act = true;
+    P->stop_mode = relres ? 1 : 2;
+    // a relative-residual test runs at the start of the next cycle
+    const int64_t cap = relres ? c.max_iters + 1 : c.max_iters;
+    std::vector<double*> starts;     // the level-0 buffer each queued cycle starts from
+    int64_t launched = 0, k = std::min<int64_t>(8, cap);
+    double v_prev = -1.0;
+    int64_t n_prev = 0;
+    int rc = 0;
+    for (;;) {
+        for (int64_t j = 0; j < k && rc == 0; ++j) {
+            starts.push_back(H.phi[0]);
+            rc = run_cycle(P, H, c, c.skip_pre != 0, s);
+        }
+        if (rc != 0) break;
+        launched += k;
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(P->h_stop, P->d_stop, sizeof(jsv::StopCtl), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (h.stopped) break;
+        if (launched >= cap) {
+            rc = fail("internal: the device stop test did not end the solve");
+            break;
+        }
+        int64_t kb = 8;
+        if (h.count > 0) kb = predict_cycles(v_prev, h.last, h.count - n_prev, h.thr, 8, 64);
+        if (relres) ++kb;            // the cycle whose start tests the last one
+        v_prev = h.last;
+        n_prev = h.count;
+        k = std::min<int64_t>(kb, cap - launched);
+    }
+    P->stop_mode = 0;
+    P->use_act = false;
+    if (rc != 0) return rc;
+    const int64_t cnt = h.count;
+    // phi after cnt cycles: the buffer the next queued cycle started from
+    // (cycles run eagerly may leave level 0's buffers swapped)
+    double* out = cnt < (int64_t)starts.size() ? starts[cnt] : H.phi[0];
+    if (out != H.phi[0]) {
+        if (out != H.phi2[0]) return fail("internal: result of the device stop test not found");
+        std::swap(H.phi[0], H.phi2[0]);
+    }
+    T.count[0] = T.outer[0] = cnt;
+    T.last[0] = h.last;
+    T.rel[0] = relres ? h.last : -1.0;
+    T.nonfinite[0] = h.nonfinite;
+    T.active = 0;
+    const long nh = std::min<long>((long)cnt, hl);
+    if (nh > 0) HIPCHK(hipMemcpy(c.history, P->d_hist, sizeof(double) * (size_t)nh, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 static int solve_fp64(jsv_plan* P, const double* phi_in, const double* rhs, double* phi_out,
                       const jsv_config* c, jsv_result* res, int64_t* counts, double* stopvals,
                       hipStream_t s)
@@ -1911,6 +2126,8 @@ static int solve_fp64(jsv_plan* P, const double* phi_in, const double* rhs, doub
     if (c->fixed) {
         for (int64_t k = 0; k < c->max_iters && rc == 0; ++k) rc = run_cycle(P, H, *c, c->skip_pre != 0, s);
         for (int b = 0; b < B; ++b) T.count[b] = T.outer[b] = c->max_iters;
+    } else if (device_stop_applies(P, c, relres, B)) {
+        rc = solve_device_stop(P, *c, relres, relres ? rhs_norm[0] : 1.0, first_old, T, s);
     } else {
         const double thr = relres ? c->rtol : c->epsilon;
         for (int b = 0; b < B; ++b)
@@ -2395,7 +2612,8 @@ void adj_drop_graphs(const jsv_plan* P)
 {
     auto it = g_adj.find(P);
     if (it == g_adj.end()) return;
-    for (auto& gr : it->second.graphs) (void)hipGraphExecDestroy(gr.exec);
+    for (auto& gr : it->second.graphs)
+        if (gr.exec != nullptr) (void)hipGraphExecDestroy(gr.exec);
     it->second.graphs.clear();
 }
 

@@ -151,8 +151,9 @@ template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_rb3(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
       const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc, Geo g, int par,
-      int z0)
+      int z0, const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     const uint3 bk = xcd_block();
     const int y = 2 + bk.y * BY + threadIdx.y;
     const int z = z0 + bk.z;
@@ -305,8 +306,9 @@ template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_rb3_pro(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ cx,
           const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc, Geo g, int par,
-          const T* __restrict__ pc, Geo gc)
+          const T* __restrict__ pc, Geo gc, const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     const uint3 bk = xcd_block();
     const int y = 2 + bk.y * BY + threadIdx.y;
     const int z = 2 + bk.z;
@@ -386,8 +388,9 @@ template <typename T, int TY, int NT>
 __global__ void __launch_bounds__(NT)
 k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
            const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-           const T* __restrict__ cc, Geo g, int zchunk, int ytiles)
+           const T* __restrict__ cc, Geo g, int zchunk, int ytiles, const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     constexpr int PR = TY + 4;                  // rows of the phi_in ring: y0-2 .. y0+TY+1
     constexpr int RR = TY + 2;                  // rows of the red ring:    y0-1 .. y0+TY
     constexpr int NHMAX = 258;                  // compact columns covered by the register prefetch (m <= 512)
@@ -525,6 +528,18 @@ k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict_
 // sweep reads).  Requires (TY + 2) * (mx / 2) <= NT, (nx + 1) / 2 <= NHMAX
 // and the level below 4 GiB; rim values of phi_in are zero.
 // ---------------------------------------------------------------------------
+// sums / maxima over the 64 lanes of a wave (every lane gets the result)
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // N consecutive elements, loaded as one dword-aligned vector access
 template <typename T, int N>
 struct __attribute__((aligned(sizeof(T)))) PairVec {
@@ -548,12 +563,16 @@ struct PairGrp {
     PairVec<T, 2 * NP> cx, cy, cym, cz, rh, dg;   // at x = 2k .. 2k + 2NP - 1; cym: row y - 1
 };
 
-template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int NP = 1, int WPE = 1, bool WR = false>
+template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int NP = 1, int WPE = 1, bool WR = false,
+          bool RES = false>
 __global__ void __launch_bounds__(NT, WPE)
 k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-            const T* __restrict__ cc, Geo g, int zchunk, int ytiles, const T* __restrict__ pc, Geo gc)
+            const T* __restrict__ cc, Geo g, int zchunk, int ytiles, const T* __restrict__ pc, Geo gc,
+            const int* __restrict__ act, double* __restrict__ res_part)
 {
+    if (act != nullptr && act[0] == 0) return;
+    static_assert(!(RES && PRO), "the residual is of phi_in as given");
     constexpr int PR = TY + 4;                  // rows of the phi_in ring: y0-2 .. y0+TY+1
     constexpr int RR = TY + 2;                  // rows of the red ring / pair rows: y0-1 .. y0+TY
     constexpr int PMAX = (PR * NHMAX + NT - 1) / NT;   // ring elements per thread (nh <= NHMAX)
@@ -569,6 +588,8 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     T* R = P + 3 * PR * nh;                     // [3][RR][nh]
     T* Hb = R + 3 * RR * nh;                    // [NP][8][NT] this thread's held black halves
     T* Cr = Hb + 8 * NP * NT;                   // PRO: [3][CRR][gc.nx] coarse planes of the correction
+    T* Ro = Cr + (PRO ? 3 * CRR * (NHMAX + 1) : 0);   // RES: [3][RR][nh] red values of phi_in
+    T* Pb = Ro + 3 * RR * NHMAX;                // RES: [NP][NT] this thread's held black values of phi_in
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int y0 = 2 + (tile % ytiles) * TY;
@@ -578,6 +599,8 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     const long sy = g.sy, sz = g.sz;
 
     for (int i = tid; i < 3 * RR * nh; i += NT) R[i] = (T)0;
+    if constexpr (RES)
+        for (int i = tid; i < 3 * RR * nh; i += NT) Ro[i] = (T)0;
 
     const bool has = tid < RR * gpr;
     const int r = has ? tid / gpr : 0;
@@ -744,6 +767,17 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         c.rh = atv(rhs, q);
         c.dg = atv(cc, q);
     };
+    // RES: the residual rhs - L phi_in at every unknown of the tile, in the
+    // term order of resid3, summed as r^2 in FP64 (the first sweep of a cycle
+    // evaluates the stop rule's residual of the cycle's input on the way)
+    double rsum = 0.0;
+    auto Rov = [&](int z, int yy, int x) -> T {
+        return Ro[((z % 3) * RR + (yy - y0 + 1)) * nh + (x >> 1)];
+    };
+    // phi_in at the black point of the previous plane whose update runs one
+    // step later, held by its thread like its coefficients (the ring slot of
+    // that plane is refilled by the next step's commit before any barrier)
+    auto Pbv = [&](int pp) -> T& { return Pb[pp * NT + tid]; };
 
     int chave = 0;                              // PRO: highest coarse plane in the ring
     if constexpr (PRO) {
@@ -786,6 +820,10 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         }
         issue(zs + 2);
         load(zs + 1, nxt);
+        // RES: phi_in of the thread's pairs on this plane (the ring fetched
+        // these cache lines two steps ago); the barrier covers the latency
+        V ph;
+        if constexpr (RES) ph = atv(pin, (unsigned)zs * szb + rowb);
         __syncthreads();
         // ---- red on plane zs (x + y + z odd); pair pp covers x = 2(k+pp), 2(k+pp)+1
         const int e = decltype(ec)::at(y, zs);  // red element within a pair: 0 when y + zs is odd
@@ -816,8 +854,24 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                 acc = acc + a_xm * p_xm;
                 acc = acc - JSV_EL(cur.rh, pp, e);
                 red[pp] = acc / JSV_EL(cur.dg, pp, e);
+                if constexpr (RES) {
+                    if (inner && zs >= z0 && zs < z1) {
+                        T rr = JSV_EL(cur.dg, pp, e) * JSV_EL(ph, pp, e);
+                        rr = rr - a_xm * p_xm;
+                        rr = rr - a_xp * p_xp;
+                        rr = rr - a_ym * p_ym;
+                        rr = rr - a_yp * p_yp;
+                        rr = rr - a_zm * p_zm;
+                        rr = rr - a_zp * p_zp;
+                        rr = rr + JSV_EL(cur.rh, pp, e);
+                        rsum += (double)rr * (double)rr;
+                    }
+                }
             }
             if (has) R[((zs % 3) * RR + r) * nh + k + pp] = red[pp];
+            // RES: phi_in's red values of plane zs, for the black points' residuals
+            if constexpr (RES)
+                if (has) Ro[((zs % 3) * RR + r) * nh + k + pp] = ld ? JSV_EL(ph, pp, e) : (T)0;
         }
         __syncthreads();
         // ---- black on plane zb = zs - 1 at x = xr; write both colours of the pairs
@@ -843,6 +897,20 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                     acc = acc + H(pp, 5) * r_xm;
                     acc = acc - H(pp, 6);
                     blk = acc / H(pp, 7);
+                    if constexpr (RES) {
+                        const T o_zp = Rov(zb + 1, y, xr), o_zm = Rov(zb - 1, y, xr);
+                        const T o_yp = Rov(zb, y + 1, xr), o_ym = Rov(zb, y - 1, xr);
+                        const T o_xp = Rov(zb, y, xr + 1), o_xm = Rov(zb, y, xr - 1);
+                        T rr = H(pp, 7) * Pbv(pp);
+                        rr = rr - H(pp, 5) * o_xm;
+                        rr = rr - H(pp, 4) * o_xp;
+                        rr = rr - H(pp, 3) * o_ym;
+                        rr = rr - H(pp, 2) * o_yp;
+                        rr = rr - H(pp, 1) * o_zm;
+                        rr = rr - H(pp, 0) * o_zp;
+                        rr = rr + H(pp, 6);
+                        rsum += (double)rr * (double)rr;
+                    }
                 }
                 const T rp = ((xr ^ 1) <= g.mx) ? red_prev[pp] : (T)0;
                 ov.v[2 * pp] = e ? rp : blk;
@@ -862,6 +930,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
             H(pp, 5) = JSV_XM(cur, pp, b1);
             H(pp, 6) = JSV_EL(cur.rh, pp, b1);
             H(pp, 7) = JSV_EL(cur.dg, pp, b1);
+            if constexpr (RES) Pbv(pp) = JSV_EL(ph, pp, b1);
             red_prev[pp] = red[pp];
         }
 #undef JSV_EL
@@ -895,6 +964,19 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         for (int zs = z0 - 1; zs <= z1; zs += 2) {
             step(zs, A, B, PairElemRT{});
             if (zs + 1 <= z1) step(zs + 1, B, A, PairElemRT{});
+        }
+    }
+    if constexpr (RES) {
+        // one partial sum per workgroup, in a fixed order
+        const double v = wave_sum(rsum);
+        __syncthreads();                        // the rings are no longer read
+        double* sh = reinterpret_cast<double*>(jsv_dyn_smem);
+        if ((tid & 63) == 0) sh[tid >> 6] = v;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int i = 0; i < NT / 64; ++i) t += sh[i];
+            res_part[blockIdx.x] = t;
         }
     }
 }
@@ -1062,8 +1144,10 @@ template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-            const T* __restrict__ cc, Geo gf, Geo gc, T* __restrict__ zphi, RedZ<T> rz)
+            const T* __restrict__ cc, Geo gf, Geo gc, T* __restrict__ zphi, RedZ<T> rz,
+            const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     const int I = 2 + blockIdx.z;
     const int J = 2 + blockIdx.y * BY + threadIdx.y;
     const int K = 2 + blockIdx.x * BX + threadIdx.x;
@@ -1095,8 +1179,9 @@ template <typename T, int RB = BY>
 __global__ void __launch_bounds__(BX * RB)
 k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
                const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-               const T* __restrict__ cc, Geo g, int remap)
+               const T* __restrict__ cc, Geo g, int remap, const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     // remap: XCD-slab block order (pays on levels the Infinity Cache holds;
     // on larger levels the launch order streams HBM better)
     const uint3 bk = remap ? xcd_block() : make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
@@ -1112,8 +1197,9 @@ k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_restrict_gather3(T* __restrict__ rc, const T* __restrict__ r, Geo gf, Geo gc, T* __restrict__ zphi,
-                   RedZ<T> rz)
+                   RedZ<T> rz, const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     const uint3 bk = xcd_block();
     const int I = 2 + bk.z;
     const int J = 2 + bk.y * BY + threadIdx.y;
@@ -1208,8 +1294,9 @@ k_prolong2(T* __restrict__ phi, const T* __restrict__ pcb, Geo gf, Geo gc, int f
 
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
-k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int full)
+k_prolong3(T* __restrict__ phi, const T* __restrict__ pc, Geo gf, Geo gc, int full, const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     const int z = 2 + blockIdx.z;
     const int y = 2 + blockIdx.y * BY + threadIdx.y;
     const int x = 2 + blockIdx.x * BX + threadIdx.x;
@@ -1761,16 +1848,6 @@ k_stage_image(T* __restrict__ img, long dbs, int dst, const T* __restrict__ src,
 // ---------------------------------------------------------------------------
 // Reductions.  Wave64 shuffles, then LDS across the 4 waves of a workgroup.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 // max |a - b| over n elements (distance, solver_2d.py:481-494).  Non-negative
 // doubles order like their bit patterns, so a 64-bit atomicMax on the bits is
@@ -1807,14 +1884,16 @@ k_maxabsdiff(const T* __restrict__ a0, const T* __restrict__ b0, long n, unsigne
 
 // Residual rhs - L phi over the unknowns of the fine level, in the arithmetic
 // type T of the operator; optionally stores it (as TO) and writes per-block
-// partial sums of r^2 (FP64) for a deterministic two-pass norm.
+// partial sums of r^2 (FP64) for a deterministic two-pass norm.  act: one
+// activity flag for the whole launch (nullptr: always runs).
 template <typename T, typename TP, typename TO>
 __global__ void __launch_bounds__(BX * BY)
 k_residual(const TP* __restrict__ phi, const TP* __restrict__ rhs, const T* __restrict__ cx,
            const T* __restrict__ cy, const T* __restrict__ cz, const T* __restrict__ cc,
            const T* __restrict__ drr, const T* __restrict__ dlr, Geo g, TO* __restrict__ rout,
-           double* __restrict__ partials, int dim)
+           double* __restrict__ partials, int dim, const int* __restrict__ act)
 {
+    if (act != nullptr && act[0] == 0) return;
     __shared__ double sh[BY];
     const int z = (dim == 3) ? 2 + (int)blockIdx.z : (int)blockIdx.z;
     const int y = 2 + blockIdx.y * BY + threadIdx.y;
@@ -1862,14 +1941,11 @@ k_norm2_unknowns(const T* __restrict__ a, Geo g, double* __restrict__ partials, 
     }
 }
 
-// Second pass: workgroup g sums partials [g*n, (g+1)*n) in a fixed order
-// (one segment per batched problem, or one for a whole 3D level).
-__global__ void __launch_bounds__(1024)
-k_sum_partials(const double* __restrict__ partials, long n, double* __restrict__ out)
+// Sum of seg[0, n) in a fixed order by a workgroup of 1024 threads; the
+// result is valid in thread 0.
+__device__ __forceinline__ double block_sum_fixed(const double* __restrict__ seg, long n, double* sh)
 {
-    __shared__ double sh[16];
     double s = 0.0;
-    const double* seg = partials + (long)blockIdx.x * n;
     // each thread adds its elements in increasing index order; eight loads are
     // issued before their adds, so the serial sum waits on one memory latency
     // per eight elements instead of one per element
@@ -1888,11 +1964,133 @@ k_sum_partials(const double* __restrict__ partials, long n, double* __restrict__
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
+    double t = 0.0;
+    if (threadIdx.x == 0)
         for (int i = 0; i < 16; ++i) t += sh[i];
-        out[blockIdx.x] = t;
+    return t;
+}
+
+// Second pass: workgroup g sums partials [g*n, (g+1)*n) in a fixed order
+// (one segment per batched problem, or one for a whole 3D level).
+__global__ void __launch_bounds__(1024)
+k_sum_partials(const double* __restrict__ partials, long n, double* __restrict__ out)
+{
+    __shared__ double sh[16];
+    const double t = block_sum_fixed(partials + (long)blockIdx.x * n, n, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// ---------------------------------------------------------------------------
+// Stop rule on the device (solver_2d.py:464-477).  A dynamic solve of one
+// problem replays its cycles with the test inside: a one-workgroup kernel
+// evaluates the stop quantity, counts the cycle and, on a stop, clears the
+// activity flag that every kernel of the solve checks first, so the cycles
+// queued behind it do nothing and the host reads the outcome once per block
+// of cycles instead of after every cycle.
+// ---------------------------------------------------------------------------
+struct StopCtl {
+    double thr;              // tolerance: rtol (relative residual) or epsilon (distance)
+    double rhs_norm;         // ||rhs|| over the unknowns (relative residual)
+    double last;             // stop value of the last tested cycle
+    long long max_iters;     // cycle limit
+    long long cycle;         // kernels of the test run so far (one per cycle)
+    long long count;         // cycles tested
+    int stopped;
+    int nonfinite;           // a stop value was NaN or infinite
+    double* hist;            // stop value of cycle k at hist[k - 1] (nullptr: not kept)
+    long long hist_len;
+};
+
+// mode 0 (relative residual, solver_2d.py rel. residual rule): runs at the
+//   start of a cycle; partials[0, n) hold the squared residual of the
+//   iterate the cycle starts from, summed in block_sum_fixed's order, so the
+//   value has the bits of the host's check.  The first cycle starts from the
+//   initial guess, which is not tested.
+// mode 1 (distance, solver_2d.py:481-494): runs at the end of a cycle;
+//   partials[0, n) hold per-workgroup maxima of |phi - phi_old| as bit
+//   patterns (k_stop_maxdiff).
+// A cycle is the last when its value is not above the tolerance (NaN
+// included: `dist > eps` is false) or the cycle limit is reached.
+__global__ void __launch_bounds__(1024)
+k_stop_check(const double* __restrict__ partials, long n, int mode, StopCtl* __restrict__ ctl,
+             int* __restrict__ act)
+{
+    if (act[0] == 0) return;
+    __shared__ double sh[16];
+    __shared__ unsigned long long shm[16];
+    double v = 0.0;
+    if (mode == 0) {
+        v = block_sum_fixed(partials, n, sh);
+    } else {
+        const unsigned long long* pu = reinterpret_cast<const unsigned long long*>(partials);
+        unsigned long long m = 0ull;
+        for (long i = threadIdx.x; i < n; i += 1024) m = pu[i] > m ? pu[i] : m;
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long u = __shfl_xor(m, o, 64);
+            m = u > m ? u : m;
+        }
+        if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long mm = shm[0];
+            for (int i = 1; i < 16; ++i) mm = shm[i] > mm ? shm[i] : mm;
+            v = __longlong_as_double((long long)mm);
+        }
     }
+    if (threadIdx.x != 0) return;
+    const long long c = ctl->cycle + 1;
+    ctl->cycle = c;
+    long long k = c;
+    if (mode == 0) {
+        if (c == 1) return;
+        v = sqrt(v) / ctl->rhs_norm;
+        k = c - 1;
+    }
+    ctl->last = v;
+    ctl->count = k;
+    if (!isfinite(v)) ctl->nonfinite = 1;
+    if (ctl->hist != nullptr && k - 1 < ctl->hist_len) ctl->hist[k - 1] = v;
+    if (!(v > ctl->thr) || k >= ctl->max_iters) {
+        ctl->stopped = 1;
+        act[0] = 0;
+    }
+}
+
+// Per-workgroup max |a - b| over n elements, as the bit pattern of the
+// non-negative double (see k_maxabsdiff), into part[blockIdx.x].
+__global__ void __launch_bounds__(256)
+k_stop_maxdiff(const double* __restrict__ a, const double* __restrict__ b, long n, double* __restrict__ part,
+               const int* __restrict__ act)
+{
+    if (act[0] == 0) return;
+    __shared__ unsigned long long sh[4];
+    unsigned long long best = 0ull;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double d = fabs(a[i] - b[i]);
+        const unsigned long long u = (unsigned long long)__double_as_longlong(d);
+        best = u > best ? u : best;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long u = __shfl_xor(best, o, 64);
+        best = u > best ? u : best;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = sh[0];
+        for (int i = 1; i < 4; ++i) m = sh[i] > m ? sh[i] : m;
+        reinterpret_cast<unsigned long long*>(part)[blockIdx.x] = m;
+    }
+}
+
+// dst := src (n elements) at the start of every cycle but the solve's first,
+// whose phi_old is the caller's initial phi, put there by the host.
+__global__ void __launch_bounds__(256)
+k_stop_copy(double* __restrict__ dst, const double* __restrict__ src, long n, const StopCtl* __restrict__ ctl,
+            const int* __restrict__ act)
+{
+    if (act[0] == 0 || ctl->cycle == 0) return;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
