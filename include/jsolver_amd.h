@@ -149,7 +149,11 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         every test.  Count and phi are the same either way;
  *                         a relative residual summed inside the first sweep
  *                         of a cycle may differ from the host's sum in the
- *                         last bits */
+ *                         last bits
+ *   JSV_TUNE_COARSE_REG   1 (default): in 2D without D_xy, the coarse
+ *                         program's one-wave runs on levels of at most 64
+ *                         points keep those levels in registers (neighbours
+ *                         by lane shuffles); 0: in LDS */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -166,6 +170,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_COARSE_THREADS 13
 #define JSV_TUNE_FUSE_HALVES 14
 #define JSV_TUNE_DEVICE_STOP 15
+#define JSV_TUNE_COARSE_REG 16
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

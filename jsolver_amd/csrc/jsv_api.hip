@@ -131,6 +131,7 @@ struct jsv_plan {
     std::vector<const void*> lv_host[2];           // pointers lv_dev holds (start level + table)
     int coarse_lds = 1;             // stage the program's coarsest levels in LDS
     int coarse_solo = jsv::COARSE_WAVE;   // ops on staged levels with at most this many points run in one wave
+    int coarse_reg = 1;             // 2D runs of such ops keep their levels' points in registers
     int coarse_threads = jsv::COARSE_THREADS;   // workgroup size of the coarse program
     int clds_from[2] = {0, 0}, clds_to[2] = {0, 0};   // staged level range per precision
     size_t clds_bytes[2] = {0, 0};
@@ -140,7 +141,7 @@ struct jsv_plan {
     std::vector<const void*> img_key[2];     // layout and operator version the image holds
     long op_version = 0;                     // bumped by every jsv_plan_set_operator
     struct CoarseProg {
-        int prec, gamma, nu1, nu2, fcyc, start, offdiag, lds_from, solo;
+        int prec, gamma, nu1, nu2, fcyc, start, offdiag, lds_from, solo, reg;
         unsigned* ops;              // packed op words; nullptr: too long, the cycle recurses instead
         int nops;
     };
@@ -714,7 +715,7 @@ const jsv_plan::CoarseProg* find_prog(const jsv_plan* P, const jsv_config& c, bo
     for (const auto& pr : P->progs)
         if (pr.prec == prec && pr.gamma == c.gamma && pr.nu1 == c.nu1 && pr.nu2 == c.nu2 &&
             pr.fcyc == (fcyc ? 1 : 0) && pr.start == start && pr.offdiag == (P->offdiag ? 1 : 0) &&
-            pr.lds_from == P->clds_from[pi] && pr.solo == P->coarse_solo)
+            pr.lds_from == P->clds_from[pi] && pr.solo == P->coarse_solo && pr.reg == P->coarse_reg)
             return &pr;
     return nullptr;
 }
@@ -886,7 +887,8 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         std::vector<jsv::CoarseOp>& ops = ops_of[fc];
         if (ops.size() > (size_t)jsv::COARSE_MAX_OPS) {
             P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
-                                P->offdiag ? 1 : 0, lds_from, P->coarse_solo, nullptr, (int)ops.size()});
+                                P->offdiag ? 1 : 0, lds_from, P->coarse_solo, P->coarse_reg, nullptr,
+                                (int)ops.size()});
             continue;
         }
         // Ops that change nothing go: smoothing halves without points, and on
@@ -940,11 +942,30 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
             }
             ops.swap(merged);
         }
+        // a run keeps its levels' points in registers when it touches at most
+        // three consecutive 2D levels of at most 64 points (padded), and no
+        // D_xy sub-colours
+        auto reg_ok = [&](size_t i, size_t k) {
+            if (!P->coarse_reg || P->dim != 2 || P->offdiag) return false;
+            int lmin = 1 << 30, lmax = -1;
+            for (size_t j = i; j < k; ++j) {
+                const jsv::CoarseOp& q = ops[j];
+                if (q.kind == jsv::CO_RB_DIAG) return false;
+                const int hi = (q.kind == jsv::CO_RESTRICT || q.kind == jsv::CO_PROLONG) ? q.level + 1 : q.level;
+                lmin = std::min(lmin, q.level);
+                lmax = std::max(lmax, hi);
+            }
+            if (lmax - lmin > 2) return false;
+            for (int l = lmin; l <= lmax; ++l)
+                if (!staged(l) || (long)H.g[l].mx * H.g[l].my > jsv::COARSE_WAVE) return false;
+            return true;
+        };
         for (size_t i = 0; i < ops.size();) {
             size_t k = i;
             while (k < ops.size() && solo(ops[k])) ++k;
             if (k > i) {
                 ops[i].run = (int)(k - i);
+                ops[i].reg = reg_ok(i, k) ? 1 : 0;
                 i = k;
             } else {
                 ++i;
@@ -956,7 +977,7 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         RETCHK(dalloc(P, (void**)&d, sizeof(unsigned) * words.size()));
         HIPCHK(hipMemcpy(d, words.data(), sizeof(unsigned) * words.size(), hipMemcpyHostToDevice));
         P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
-                            P->offdiag ? 1 : 0, lds_from, P->coarse_solo, d, (int)ops.size()});
+                            P->offdiag ? 1 : 0, lds_from, P->coarse_solo, P->coarse_reg, d, (int)ops.size()});
     }
     return 0;
 }
@@ -1484,6 +1505,10 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     }
     if (key == JSV_TUNE_DEVICE_STOP) {
         P->device_stop = value != 0 ? 1 : 0;
+        return 0;
+    }
+    if (key == JSV_TUNE_COARSE_REG) {
+        P->coarse_reg = value != 0 ? 1 : 0;
         return 0;
     }
     if (key == JSV_TUNE_FUSE_PROLONG) {

@@ -1360,18 +1360,20 @@ enum CoarseOpKind : int {
 struct CoarseOp {
     int kind, level, a, b;
     int run;          // > 0: first of `run` consecutive ops that wave 0 runs alone
+    int reg;          // (first op of a run) the run keeps its levels' points in registers
 };
 
 // An op as the kernel reads it: one 32-bit word (kind 3 bits, level 5, a 1,
-// b 1, run 22), so the whole list sits in LDS next to the levels.
+// b 1, run 21, reg 1), so the whole list sits in LDS next to the levels.
 __host__ __device__ __forceinline__ unsigned coarse_pack(const CoarseOp& op)
 {
     return (unsigned)op.kind | ((unsigned)op.level << 3) | ((unsigned)op.a << 8) | ((unsigned)op.b << 9) |
-           ((unsigned)op.run << 10);
+           ((unsigned)op.run << 10) | ((unsigned)op.reg << 31);
 }
 __device__ __forceinline__ CoarseOp coarse_unpack(unsigned w)
 {
-    return {(int)(w & 7u), (int)((w >> 3) & 31u), (int)((w >> 8) & 1u), (int)((w >> 9) & 1u), (int)(w >> 10)};
+    return {(int)(w & 7u), (int)((w >> 3) & 31u), (int)((w >> 8) & 1u), (int)((w >> 9) & 1u),
+            (int)((w >> 10) & 0x1fffffu), (int)(w >> 31)};
 }
 constexpr int COARSE_MAX_OPS = 8192;   // longer programs run as separate launches
 constexpr int COARSE_THREADS = 1024;
@@ -1662,6 +1664,188 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register runs (2D without D_xy): a run of one-wave ops on levels of at most
+// 64 points each (padded: indices 2..m+1 per axis) keeps every point of up to
+// three consecutive levels in a lane's registers, lane = yi * mx + xi for the
+// point (2 + yi, 2 + xi); the last column and row are the boundary, whose phi
+// stays zero.  Neighbours arrive from other lanes by shuffles instead of LDS
+// round trips.  The arithmetic is rb2_point_pre / restrict2_at_pre /
+// prolong2_corr_at's, in the same order, so the bits are the LDS ops'.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RegLv {
+    T phi, rhs, cxp, cxm, cyp, cym, cc;
+    int mx, my, lmx;
+};
+
+template <typename T>
+__device__ __forceinline__ void reg_load(RegLv<T>& S, const LvPtr<T>& L, T* sm, int lane)
+{
+    S.mx = L.g.mx;
+    S.my = L.g.my;
+    S.lmx = __builtin_ctz((unsigned)S.mx);
+    const LvView<T, true> V(L, sm, 0, 2);
+    const bool in = lane < S.mx * S.my;
+    const int xi = lane & (S.mx - 1), yi = lane >> S.lmx;
+    const long p = in ? V.at(2 + yi, 2 + xi) : V.at(2, 2);
+    const T a0 = V.phi[p], a1 = V.rhs[p], a2 = V.cx[p], a3 = V.cx[p - 1], a4 = V.cy[p], a5 = V.cy[p - V.sy],
+            a6 = V.cc[p];
+    S.phi = in ? a0 : (T)0;
+    S.rhs = a1;
+    S.cxp = a2;
+    S.cxm = a3;
+    S.cyp = a4;
+    S.cym = a5;
+    S.cc = a6;
+}
+
+template <typename T>
+__device__ __forceinline__ void reg_store(const RegLv<T>& S, const LvPtr<T>& L, T* sm, int lane)
+{
+    const LvView<T, true> V(L, sm, 0, 2);
+    if (lane < S.mx * S.my) {
+        const long p = V.at(2 + (lane >> S.lmx), 2 + (lane & (S.mx - 1)));
+        V.phi[p] = S.phi;
+        V.rhs[p] = S.rhs;
+    }
+}
+
+// phi at the four neighbours of this lane's point; 0 past the low boundary
+// (index 1), the high boundary lanes hold 0
+template <typename T>
+__device__ __forceinline__ void reg_nbrs(const RegLv<T>& S, int lane, T& pxp, T& pxm, T& pyp, T& pym)
+{
+    const int xi = lane & (S.mx - 1), yi = lane >> S.lmx;
+    pxp = __shfl(S.phi, (lane + 1) & 63, 64);
+    pxm = __shfl(S.phi, (lane - 1) & 63, 64);
+    pyp = __shfl(S.phi, (lane + S.mx) & 63, 64);
+    pym = __shfl(S.phi, (lane - S.mx) & 63, 64);
+    if (xi == 0) pxm = (T)0;
+    if (yi == 0) pym = (T)0;
+}
+
+template <typename T>
+__device__ __forceinline__ void reg_rb(RegLv<T>& S, int par, int lane)
+{
+    const int xi = lane & (S.mx - 1), yi = lane >> S.lmx;
+    const bool upd = lane < S.mx * S.my && xi <= S.mx - 2 && yi <= S.my - 2 && ((xi + yi) & 1) == par;
+    T pxp, pxm, pyp, pym;
+    reg_nbrs(S, lane, pxp, pxm, pyp, pym);
+    T acc = S.cyp * pyp + S.cym * pym;
+    acc = acc + S.cxp * pxp;
+    acc = acc + S.cxm * pxm;
+    acc = acc - S.rhs;
+    const T v = acc / S.cc;
+    if (upd) S.phi = v;
+}
+
+template <typename T>
+__device__ __forceinline__ void reg_restrict(const RegLv<T>& F, RegLv<T>& C, int zero, int lane)
+{
+    T pxp, pxm, pyp, pym;
+    reg_nbrs(F, lane, pxp, pxm, pyp, pym);
+    T r = F.cc * F.phi;
+    r = r - F.cxm * pxm;
+    r = r - F.cxp * pxp;
+    r = r - F.cym * pym;
+    r = r - F.cyp * pyp;
+    r = r + F.rhs;
+    // coarse point (2 + yc, 2 + xc) weights the fine points around fine lane (2yc + 1, 2xc + 1)
+    const int xc = lane & (C.mx - 1), yc = lane >> C.lmx;
+    const int fx = 2 * xc + 1, fy = 2 * yc + 1;
+    auto R = [&](int yy, int xx) { return __shfl(r, (yy * F.mx + xx) & 63, 64); };
+    const T ra = R(fy + 1, fx + 1), rb = R(fy + 1, fx - 1), rd = R(fy - 1, fx + 1), re = R(fy - 1, fx - 1),
+            rc = R(fy, fx);
+    T sum = ra + rb;
+    sum = sum + rd;
+    sum = sum + re;
+    const T v = (T)0.0625 * sum + (T)0.25 * rc;
+    if (lane < C.mx * C.my && xc <= C.mx - 2 && yc <= C.my - 2) C.rhs = v;
+    if (zero) C.phi = (T)0;
+}
+
+template <typename T>
+__device__ __forceinline__ void reg_prolong(RegLv<T>& F, const RegLv<T>& C, int full, int lane)
+{
+    const int xi = lane & (F.mx - 1), yi = lane >> F.lmx;
+    const int x = 2 + xi, y = 2 + yi;
+    const int Xl = x >> 1, Yl = y >> 1;
+    // coarse phi at coarse point (J, K); index 1 is the boundary (zero)
+    auto Cv = [&](int J, int K) -> T {
+        const bool ok = J >= 2 && K >= 2;
+        const T v = __shfl(C.phi, ok ? ((J - 2) * C.mx + (K - 2)) & 63 : 0, 64);
+        return ok ? v : (T)0;
+    };
+    const T A = Cv(Yl + 1, Xl + 1), B = Cv(Yl, Xl + 1), Cc = Cv(Yl + 1, Xl), D = Cv(Yl, Xl);
+    const int ox = x & 1, oy = y & 1;
+    T corr = (T)0;
+    bool has = true;
+    if (ox && oy) {
+        corr = A;
+    } else if (oy) {
+        corr = (T)0.5 * (A + Cc);
+    } else if (ox) {
+        corr = (T)0.5 * (A + B);
+    } else if (full) {
+        T sum = A + B;
+        sum = sum + Cc;
+        sum = sum + D;
+        corr = (T)0.25 * sum;
+    } else {
+        has = false;
+    }
+    if (lane < F.mx * F.my && xi <= F.mx - 2 && yi <= F.my - 2 && has) F.phi = F.phi + corr;
+}
+
+// The run of `run` ops from word o, in wave 0: the levels it touches (at most
+// three, lmin..lmax) are loaded into registers, the ops execute there, and
+// phi and rhs go back to LDS.
+template <typename T>
+__device__ __forceinline__ void coarse_reg_run(const unsigned* sops, int o, int run, const LvPtr<T>* __restrict__ lv,
+                                               T* sm, int lane)
+{
+    int lmin = 31, lmax = 0;
+    for (int k = 0; k < run; ++k) {
+        const CoarseOp q = coarse_unpack(__builtin_amdgcn_readfirstlane(sops[o + k]));
+        const int hi = (q.kind == CO_RESTRICT || q.kind == CO_PROLONG) ? q.level + 1 : q.level;
+        lmin = q.level < lmin ? q.level : lmin;
+        lmax = hi > lmax ? hi : lmax;
+    }
+    RegLv<T> S0, S1, S2;
+    reg_load(S0, lv[lmin], sm, lane);
+    if (lmin + 1 <= lmax) reg_load(S1, lv[lmin + 1], sm, lane);
+    if (lmin + 2 <= lmax) reg_load(S2, lv[lmin + 2], sm, lane);
+    unsigned wn = sops[o];
+    for (int k = 0; k < run; ++k) {
+        const CoarseOp q = coarse_unpack(__builtin_amdgcn_readfirstlane(wn));
+        if (k + 1 < run) wn = sops[o + k + 1];   // next op word, read ahead
+        const int j = q.level - lmin;
+        if (q.kind == CO_RB || q.kind == CO_RB_PAIR) {
+            auto sweep = [&](RegLv<T>& S) {
+                reg_rb(S, q.a, lane);
+                if (q.kind == CO_RB_PAIR) reg_rb(S, 1 - q.a, lane);
+            };
+            if (j == 0) sweep(S0);
+            else if (j == 1) sweep(S1);
+            else sweep(S2);
+        } else if (q.kind == CO_RESTRICT) {
+            if (j == 0) reg_restrict(S0, S1, q.b, lane);
+            else reg_restrict(S1, S2, q.b, lane);
+        } else if (q.kind == CO_PROLONG) {
+            if (j == 0) reg_prolong(S0, S1, q.a, lane);
+            else reg_prolong(S1, S2, q.a, lane);
+        } else {  // CO_ZERO
+            if (j == 0) S0.phi = (T)0;
+            else if (j == 1) S1.phi = (T)0;
+            else S2.phi = (T)0;
+        }
+    }
+    reg_store(S0, lv[lmin], sm, lane);
+    if (lmin + 1 <= lmax) reg_store(S1, lv[lmin + 1], sm, lane);
+    if (lmin + 2 <= lmax) reg_store(S2, lv[lmin + 2], sm, lane);
+}
+
 // Coordinates (z, y, x) of linear index i over planes of ny rows of nx,
 // advanced by a fixed stride: one division at the start, none per step.
 struct Walk3 {
@@ -1774,7 +1958,11 @@ k_coarse_prog(const LvPtr<T>* __restrict__ lv, const unsigned* __restrict__ ops,
     __syncthreads();
     for (int o = 0; o < nops;) {
         const CoarseOp op = coarse_unpack(__builtin_amdgcn_readfirstlane(sops[o]));
-        if (op.run > 0) {
+        if (op.run > 0 && dim == 2 && op.reg) {
+            if (tid < COARSE_WAVE) coarse_reg_run<T>(sops, o, op.run, lv, sm, tid);
+            // the ops after the run read what the wave stored
+            o += op.run;
+        } else if (op.run > 0) {
             if (tid < COARSE_WAVE) {
                 unsigned wn = sops[o + 1 < nops ? o + 1 : o];   // next op word, read ahead
                 for (int k = 0; k < op.run; ++k) {

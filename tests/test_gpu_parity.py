@@ -356,6 +356,31 @@ def test_cycle_coarse_program_launch_shapes(case, cyc, solo, threads):
         np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"call {rep}")
 
 
+REG_CASES = [c for c in CASES if c[0] in ("2d_spatial_16x32", "2d_spatial_32x8", "2d_scalar")] + [
+    ("2d_spatial_128x64", 2, (128, 64), True, True, False)]
+
+
+@pytest.mark.parametrize("reg", [1, 0])
+@pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4], CYCLES[5]], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", REG_CASES, ids=[c[0] for c in REG_CASES])
+def test_cycle_coarse_program_register_runs(case, cyc, reg):
+    """2D one-wave runs with their levels in registers (neighbours by lane
+    shuffles; square and rectangular levels) and in LDS: the oracle's bits."""
+    from jsolver_amd import _lib
+    lib = _lib.load()
+    dp, op, phi, rhs = _setup(case, seed=29)
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, 1 << 20))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_REG, reg))
+    kw = cyc[1]
+    c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+    want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
+    for rep in range(2):
+        dp.upload(0, 0, phi)
+        dp.upload(0, 1, rhs)
+        dp.cycle(cfg(**kw))
+        np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"call {rep}")
+
+
 def test_coarse_tuning_limits():
     from jsolver_amd import _lib
     lib = _lib.load()
