@@ -1737,7 +1737,7 @@ __device__ __forceinline__ void reg_rb(RegLv<T>& S, int par, int lane)
     acc = acc + S.cxm * pxm;
     acc = acc - S.rhs;
     const T v = acc / S.cc;
-    if (upd) S.phi = v;
+    S.phi = upd ? v : S.phi;
 }
 
 template <typename T>
@@ -1761,8 +1761,9 @@ __device__ __forceinline__ void reg_restrict(const RegLv<T>& F, RegLv<T>& C, int
     sum = sum + rd;
     sum = sum + re;
     const T v = (T)0.0625 * sum + (T)0.25 * rc;
-    if (lane < C.mx * C.my && xc <= C.mx - 2 && yc <= C.my - 2) C.rhs = v;
-    if (zero) C.phi = (T)0;
+    const bool cunk = lane < C.mx * C.my && xc <= C.mx - 2 && yc <= C.my - 2;
+    C.rhs = cunk ? v : C.rhs;
+    C.phi = zero ? (T)0 : C.phi;
 }
 
 template <typename T>
@@ -1779,23 +1780,15 @@ __device__ __forceinline__ void reg_prolong(RegLv<T>& F, const RegLv<T>& C, int 
     };
     const T A = Cv(Yl + 1, Xl + 1), B = Cv(Yl, Xl + 1), Cc = Cv(Yl + 1, Xl), D = Cv(Yl, Xl);
     const int ox = x & 1, oy = y & 1;
-    T corr = (T)0;
-    bool has = true;
-    if (ox && oy) {
-        corr = A;
-    } else if (oy) {
-        corr = (T)0.5 * (A + Cc);
-    } else if (ox) {
-        corr = (T)0.5 * (A + B);
-    } else if (full) {
-        T sum = A + B;
-        sum = sum + Cc;
-        sum = sum + D;
-        corr = (T)0.25 * sum;
-    } else {
-        has = false;
-    }
-    if (lane < F.mx * F.my && xi <= F.mx - 2 && yi <= F.my - 2 && has) F.phi = F.phi + corr;
+    // every case's value, then a select (no divergent branches)
+    const T c_y = (T)0.5 * (A + Cc), c_x = (T)0.5 * (A + B);
+    T sum = A + B;
+    sum = sum + Cc;
+    sum = sum + D;
+    const T c_e = (T)0.25 * sum;
+    const T corr = (ox && oy) ? A : oy ? c_y : ox ? c_x : c_e;
+    const bool upd = lane < F.mx * F.my && xi <= F.mx - 2 && yi <= F.my - 2 && (ox || oy || full);
+    F.phi = upd ? F.phi + corr : F.phi;
 }
 
 // The run of `run` ops from word o, in wave 0: the levels it touches (at most
