@@ -153,7 +153,12 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *   JSV_TUNE_COARSE_REG   1 (default): in 2D without D_xy, the coarse
  *                         program's one-wave runs on levels of at most 64
  *                         points keep those levels in registers (neighbours
- *                         by lane shuffles); 0: in LDS */
+ *                         by lane shuffles); 0: in LDS
+ *   JSV_TUNE_ZIGZAG       1 (default): a pair sweep without the prolongation
+ *                         or the residual sum marches its level's planes in
+ *                         the opposite order of that level's previous pair
+ *                         sweep (the Infinity Cache still holds the planes
+ *                         the previous sweep touched last); 0: always upward */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -171,6 +176,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_FUSE_HALVES 14
 #define JSV_TUNE_DEVICE_STOP 15
 #define JSV_TUNE_COARSE_REG 16
+#define JSV_TUNE_ZIGZAG 17
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

@@ -122,6 +122,8 @@ struct jsv_plan {
     int sweep_variant = 2;          // 1: thread per ring point; 2: thread per column pair, pipelined
     int sweep_pairs = 0;            // pair kernel: column pairs per thread (0: 2 in FP32, 1 in FP64)
     int n_cu = 256;                 // compute units of the device
+    int zigzag = 1;                 // a plain pair sweep marches opposite to its level's previous pair sweep
+    mutable unsigned zz_dn[2] = {0u, 0u};   // FP64 / FP32: bit l = level l's last pair sweep marched downward
     int fuse_prolong = 2;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
     int fuse_halves = 1;            // levels smoothed one launch per half: the restriction runs the coarse
                                     // level's first half, the first post-smoothing half applies the prolongation
@@ -343,8 +345,14 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
     static const bool wr_plain = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true>);
     const bool wr = ((g.mx / 2) / NP) % 64 == 0 && (pc != nullptr ? wr_pro : wr_plain);
     auto go = [&](auto kern, const T* pcv, size_t lds_k = 0, double* rp = nullptr) {
+        // zig-zag: a plain sweep reverses its level's previous pair sweep
+        // (the variants with the prolongation or the residual sum march upward)
+        unsigned& last = P->zz_dn[sizeof(T) == 8 ? 0 : 1];
+        const unsigned bit = 1u << l;
+        const int dn = (P->zigzag && pcv == nullptr && rp == nullptr && (last & bit) == 0u) ? 1 : 0;
+        last = dn ? (last | bit) : (last & ~bit);
         hipLaunchKernelGGL(kern, grid, dim3(NT), lds_k > 0 ? lds_k : lds, s, H.phi[l], H.phi2[l], H.rhs[l],
-                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp);
+                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp, dn);
     };
     if (nres != nullptr) *nres = 0;
     if constexpr (sizeof(T) == 8 && NP == 1 && (TY == 5 || TY == 4 || TY == 2)) {
@@ -1505,6 +1513,10 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     }
     if (key == JSV_TUNE_DEVICE_STOP) {
         P->device_stop = value != 0 ? 1 : 0;
+        return 0;
+    }
+    if (key == JSV_TUNE_ZIGZAG) {
+        P->zigzag = value != 0 ? 1 : 0;
         return 0;
     }
     if (key == JSV_TUNE_COARSE_REG) {

@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(NT, WPE)
 k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
             const T* __restrict__ cc, Geo g, int zchunk, int ytiles, const T* __restrict__ pc, Geo gc,
-            const int* __restrict__ act, double* __restrict__ res_part)
+            const int* __restrict__ act, double* __restrict__ res_part, int dn_in)
 {
     if (act != nullptr && act[0] == 0) return;
     static_assert(!(RES && PRO), "the residual is of phi_in as given");
@@ -615,6 +615,20 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // whose bases stay in scalar registers.
     const unsigned rowb = (unsigned)(((ld ? y : 1) * g.sy + 2 * k) * (long)sizeof(T));
     const unsigned syb = (unsigned)(g.sy * sizeof(T)), szb = (unsigned)(g.sz * sizeof(T));
+    // dn: the chunks are marched downward, so that a sweep can visit the
+    // planes in the reverse order of the pass before it (the planes that pass
+    // touched last are still in the Infinity Cache).  The kernel works in march
+    // coordinates u; plane u is stored at pl(u) = nz - 1 - u.  The mirror maps
+    // unknowns to unknowns and keeps every parity (mz + 2 is even); the z face
+    // above march plane u is the stored cz of plane pl(u + 1) (clamped for the
+    // prefetch past the last plane, whose values are never used).  The two z
+    // terms of a stencil sum swap places, and that first sum is commutative,
+    // so the bits are those of the upward march.  The variants with the
+    // prolongation or the residual sum (whose term orders are not symmetric in
+    // z) always march upward.
+    const int dn = (PRO || RES) ? 0 : dn_in;
+    auto pl = [&](int u) -> int { return dn ? g.nz - 1 - u : u; };
+    auto plz = [&](int u) -> int { return dn ? max(g.nz - 2 - u, 0) : u; };
     auto at = [](const T* base, unsigned boff) -> T {
         return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
     };
@@ -629,7 +643,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         const int yy = y0 - 2 + rr_;
         const int x = 2 * kk + ((yy + z) & 1);
         ok = i < PR * nh && z >= 0 && z < g.nz && yy >= 0 && yy < g.ny && x < g.nx;
-        return ok ? (unsigned)z * szb + (unsigned)yy * syb + (unsigned)x * W : 0u;
+        return ok ? (unsigned)pl(z) * szb + (unsigned)yy * syb + (unsigned)x * W : 0u;
     };
     auto load_plane = [&](int z, auto corr_fn) {
         T* dst = P + (z % 3) * PR * nh;
@@ -758,12 +772,12 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + boff);
     };
     auto load = [&](int z, PairGrp<T, NP>& c) {
-        const unsigned q = (unsigned)z * szb + rowb;
+        const unsigned q = (unsigned)pl(z) * szb + rowb;
         c.cxm = at(cx, q - W);
         c.cx = atv(cx, q);
         c.cy = atv(cy, q);
         c.cym = atv(cy, q - syb);
-        c.cz = atv(cz, q);
+        c.cz = atv(cz, (unsigned)plz(z) * szb + rowb);
         c.rh = atv(rhs, q);
         c.dg = atv(cc, q);
     };
@@ -791,7 +805,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     prep_corr(z0);
     PairGrp<T, NP> A, B;
     load(z0 - 1, A);
-    V czp = atv(cz, (unsigned)(z0 - 2) * szb + rowb);   // cz of the plane below the current one
+    V czp = atv(cz, (unsigned)plz(z0 - 2) * szb + rowb);   // cz of the plane below the current one
     // the black halves of the previous plane (their updates run one step
     // later) wait in LDS, private to the thread: no barrier needed
     auto H = [&](int pp, int j) -> T& { return Hb[(pp * 8 + j) * NT + tid]; };
@@ -823,7 +837,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         // RES: phi_in of the thread's pairs on this plane (the ring fetched
         // these cache lines two steps ago); the barrier covers the latency
         V ph;
-        if constexpr (RES) ph = atv(pin, (unsigned)zs * szb + rowb);
+        if constexpr (RES) ph = atv(pin, (unsigned)pl(zs) * szb + rowb);
         __syncthreads();
         // ---- red on plane zs (x + y + z odd); pair pp covers x = 2(k+pp), 2(k+pp)+1
         const int e = decltype(ec)::at(y, zs);  // red element within a pair: 0 when y + zs is odd
@@ -877,7 +891,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         // ---- black on plane zb = zs - 1 at x = xr; write both colours of the pairs
         const int zb = zs - 1;
         if (inner && zb >= z0 && zb < z1) {
-            T* orow = pout + ((long)zb * sz + (long)y * sy);
+            T* orow = pout + ((long)pl(zb) * sz + (long)y * sy);
             // both colours of the thread's pairs leave as one vector store; a
             // pair reaching past the last unknown (x = mx + 1, a boundary
             // node) writes the zero that node holds

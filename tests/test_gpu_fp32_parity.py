@@ -79,14 +79,16 @@ def test_fp32_smooth_bitexact(case):
         np.testing.assert_array_equal(dp.download(level, 0, prec=F32), want, err_msg=f"level {level}")
 
 
+@pytest.mark.parametrize("zigzag", [0, 1])
 @pytest.mark.parametrize("pairs", [1, 2])
 @pytest.mark.parametrize("ms", [(32, 16, 16), (64, 32, 16)])
-def test_fp32_fused_pair_sweep_bitexact(ms, pairs):
+def test_fp32_fused_pair_sweep_bitexact(ms, pairs, zigzag):
     """The fused out-of-place pair sweep (the 256^3 fine level's kernel),
-    forced onto small levels."""
+    forced onto small levels; with zig-zag on, one of each level's two
+    sweeps marches downward."""
     case = ("3d", 3, ms, True, True, False)
     dp, op, _, _ = _setup(case, seed=5)
-    _tune(dp, FUSED_MIN_M=2, SWEEP_PAIRS=pairs)
+    _tune(dp, FUSED_MIN_M=2, SWEEP_PAIRS=pairs, ZIGZAG=zigzag)
     for level in range(2):
         phi, rhs = _level_arrays(op, level, 20 + level)
         dp.upload(level, 0, phi, prec=F32)

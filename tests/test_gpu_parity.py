@@ -134,6 +134,37 @@ def test_fused_sweep_bitexact(ms, zchunk, variant, rows, pairs):
         np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle))
 
 
+@pytest.mark.parametrize("zigzag", [0, 1])
+@pytest.mark.parametrize("zchunk", [1, 3, 0])
+@pytest.mark.parametrize("pairs", [1, 2])
+@pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 64), (8, 8, 8)])
+def test_pair_sweep_zigzag_bitexact(ms, pairs, zchunk, zigzag):
+    """Pair sweeps that march downward (every other plain sweep of a level
+    with zig-zag on) give the oracle's bits; so do 1, 2 and 3 consecutive
+    sweeps, which start in either direction, and the cycles around them."""
+    from jsolver_amd import _lib
+    case = ("3d_zigzag", 3, ms, True, True, False)
+    dp, op, phi0, rhs0 = _setup(case, seed=17)
+    lib = _lib.load()
+    for key, v in ((_lib.JSV_TUNE_FUSED_MIN_M, 2), (_lib.JSV_TUNE_ZCHUNK, zchunk),
+                   (_lib.JSV_TUNE_SWEEP_PAIRS, pairs), (_lib.JSV_TUNE_ZIGZAG, zigzag)):
+        _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, key, v))
+    phi, rhs = _level_arrays(op, 0, 70)
+    want = phi
+    for sweeps in (1, 2, 3):
+        want = gmg.smooth(op, 0, want, rhs)
+        dp.upload(0, 0, phi)
+        dp.upload(0, 1, rhs)
+        dp.smooth(0, sweeps)
+        np.testing.assert_array_equal(dp.download(0, 0), want, err_msg=f"{sweeps} sweeps")
+    for kw in (dict(), dict(nu1=2, nu2=2), dict(gamma=2, fcycle=True)):
+        dp.upload(0, 0, phi0)
+        dp.upload(0, 1, rhs0)
+        dp.cycle(cfg(**kw))
+        c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+        np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle))
+
+
 @pytest.mark.parametrize("pairs", [1, 2])
 @pytest.mark.parametrize("fuse", [2, 0])
 @pytest.mark.parametrize("rows", [0, 2, 5])
