@@ -28,6 +28,17 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
     operator symmetry; ghost values of D/lambda/rhs have no effect.
   * residual_norm / apply_operator (new, no reference counterpart): compared
     with a scipy sparse assembly of the same operator.
+  * bcs_2d: the reference's own do_BCs tests (default, vals, x and y direction)
+    restated; bcs_3d (no reference test): hand-built expected arrays for every
+    direction and rim, including the z faces taking vals_yb / vals_ye.
+  * prolong, including the nu2 <= 0 (even, even[, even]) interpolations: every
+    fine unknown reproduces an affine field sampled on the coarse grid exactly,
+    and a variant with one face weight changed fails that check.
+  * dirichlet_lift (boundary="phi", no reference counterpart): equals a direct
+    sparse solve with the boundary values moved to the right-hand side.
+  * cast_operator (the FP32 hierarchy): the float32 results stay within FP32
+    rounding of the FP64 oracle.
+Every function has such a check; no part of the oracle is unpinned.
 """
 from __future__ import annotations
 

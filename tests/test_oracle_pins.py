@@ -13,6 +13,8 @@
   reference interpolates (3D bilinear faces only when nu2 <= 0).
 * The operator is symmetric without D_xy; ghost values of D, lambda and rhs
   have no effect.
+* The FP32 restatement (cast_operator, float32 phi and rhs) computes in
+  float32, within FP32 rounding of the FP64 oracle.
 Each check is chosen so a plausible slip (a dropped face, a shifted index, a
 missing average, a sign) fails it.
 """
@@ -505,3 +507,24 @@ def test_prolongation_affine_pin_has_power():
     inner = (slice(1, -1),) * 3
     assert np.array_equal(c[inner], want[inner])
     assert not np.array_equal(bad[inner], want[inner])
+
+
+def test_oracle_runs_in_float32():
+    """The FP32 restatement really computes in float32: float32 results, within
+    FP32 rounding of the FP64 oracle and not equal to its rounding."""
+    ms = (16, 16, 16)
+    rng = np.random.default_rng(3)
+    shape = tuple(m + 3 for m in reversed(ms))
+    D = [np.exp(0.5 * rng.standard_normal(shape)) for _ in range(3)]
+    lam = rng.uniform(0.0, 2.0, shape)
+    rhs = rng.standard_normal(shape)
+    phi = rng.standard_normal(shape)
+    op64 = gmg.build_operator(3, ms, tuple(g.idel for g in _grids(ms)), lam, D)
+    op32 = gmg.cast_operator(op64, np.float32)
+    p32, r32 = phi.astype(np.float32), rhs.astype(np.float32)
+    c = gmg.Config(fixed=True, max_iters_outer=1)
+    a = gmg.cycle(op32, c, gmg.zero_rim(p32), r32)
+    b = gmg.cycle(op64, c, gmg.zero_rim(p32.astype(np.float64)), r32.astype(np.float64))
+    assert a.dtype == np.float32
+    assert np.max(np.abs(a - b)) <= 1e-4 * np.max(np.abs(b))
+    assert not np.array_equal(a, b.astype(np.float32))
