@@ -30,7 +30,9 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--inner-cycles", type=int, default=0,
                     help="FP32 cycles per refinement step (0: predicted from the contraction)")
-    ap.add_argument("--sweeps", type=int, default=20, help="sweeps per roofline timing")
+    ap.add_argument("--sweeps", type=int, default=20, help="sweeps per timed block of the roofline timing")
+    ap.add_argument("--sweep-blocks", type=int, default=7,
+                    help="timed blocks per level; the median block gives the sweep time")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cycles", type=int, default=8,
                     help="minimum V-cycles of the C oracle timed for the CPU baseline (more are run "
@@ -46,9 +48,11 @@ def log(msg):
 
 def roofline(a, grids, m):
     """Per-level RBGS sweep time with HIP events on the stream the kernels run
-    on; the fine level's FP32 sweep is the roofline kernel.  Algorithmic bytes
-    per sweep and unknown: 7 words (phi read + write, rhs, 3 face arrays, the
-    diagonal)."""
+    on (median of several blocks of back-to-back sweeps; consecutive sweeps
+    of a level march in opposite z directions, as in a cycle's post- and
+    pre-smoothing); the fine level's FP32 sweep is the roofline kernel.
+    Algorithmic bytes per sweep and unknown: 7 words (phi read + write, rhs,
+    3 face arrays, the diagonal)."""
     import torch
 
     from jsolver_amd import _lib, _solver
@@ -62,12 +66,16 @@ def roofline(a, grids, m):
             n_unk = (mm - 1) ** 3
             sweeps = a.sweeps if level < 3 else 50
             _lib.check(lib.jsv_smooth(plan.h, prec, level, 2, stream_ptr()))
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            _lib.check(lib.jsv_smooth(plan.h, prec, level, sweeps, stream_ptr()))
-            e1.record()
-            torch.cuda.synchronize()
-            t_sweep = e0.elapsed_time(e1) / 1e3 / sweeps
+            # median over blocks of `sweeps` back-to-back sweeps
+            blocks = []
+            for _ in range(a.sweep_blocks):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                _lib.check(lib.jsv_smooth(plan.h, prec, level, sweeps, stream_ptr()))
+                e1.record()
+                torch.cuda.synchronize()
+                blocks.append(e0.elapsed_time(e1) / 1e3 / sweeps)
+            t_sweep = sorted(blocks)[len(blocks) // 2]
             gbs = 7 * w * n_unk / t_sweep / 1e9
             per_level.append({"prec": prec_name, "level": level, "m": mm,
                               "us_per_sweep": round(t_sweep * 1e6, 2), "GBps": round(gbs, 1)})
