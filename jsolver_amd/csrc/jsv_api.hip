@@ -143,7 +143,7 @@ struct jsv_plan {
     std::vector<const void*> img_key[2];     // layout and operator version the image holds
     long op_version = 0;                     // bumped by every jsv_plan_set_operator
     struct CoarseProg {
-        int prec, gamma, nu1, nu2, fcyc, start, offdiag, lds_from, solo, reg;
+        int prec, gamma, nu1, nu2, fcyc, start, offdiag, lds_from, solo, reg, fuse;
         unsigned* ops;              // packed op words; nullptr: too long, the cycle recurses instead
         int nops;
     };
@@ -686,33 +686,58 @@ int coarse_start(const jsv_plan* P, const Hier<T>& H)
     return -1;
 }
 
-// the op list of mg_cycle(l, fcyc, skip_pre = false), in mg_cycle's order
-void build_ops(const jsv_plan* P, const jsv_config& c, int l, bool fcyc, std::vector<jsv::CoarseOp>& ops)
+// the op list of mg_cycle(l, fcyc, skip_pre = false), in mg_cycle's order.
+// With fuse_halves (2D without D_xy) the launch levels' two fusions apply here
+// too: the restriction into level l + 1 also runs the first half of the first
+// sweep of its first visit (phi = 0 there), and the prolongation runs inside
+// the first post-smoothing half (nu2 > 0).  red_done: the restriction into
+// level l already ran the first half of this visit's first sweep.  In 3D the
+// fused post-smoothing half (six corrected neighbours per point, two to eight
+// coarse values each) made the program slower (FP32 34 -> 39 us per launch)
+// and the fused restriction gained nothing, so 3D programs keep separate ops.
+bool op_empty(const jsv_plan* P, const std::vector<Geo>& G, const jsv::CoarseOp& op);
+
+void build_ops(const jsv_plan* P, const std::vector<Geo>& G, const jsv_config& c, int l, bool fcyc,
+               std::vector<jsv::CoarseOp>& ops, bool red_done = false)
 {
-    auto smooth_ops = [&](int lv) {
-        if (P->dim == 3) {
-            ops.push_back({jsv::CO_RB, lv, 1, 0});
-            ops.push_back({jsv::CO_RB, lv, 0, 0});
-        } else if (P->offdiag) {
+    const bool fuse = P->fuse_halves && P->dim == 2 && !P->offdiag;
+    const int first_par = P->dim == 3 ? 1 : 0;    // parity of a sweep's first half
+    auto smooth_ops = [&](int lv, bool skip_first) {
+        if (P->dim == 2 && P->offdiag) {
             static const int order[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
             for (int k = 0; k < 4; ++k) ops.push_back({jsv::CO_RB_DIAG, lv, order[k][0], order[k][1]});
-        } else {
-            ops.push_back({jsv::CO_RB, lv, 0, 0});
-            ops.push_back({jsv::CO_RB, lv, 1, 0});
+            return;
         }
+        if (!skip_first) ops.push_back({jsv::CO_RB, lv, first_par, 0});
+        ops.push_back({jsv::CO_RB, lv, 1 - first_par, 0});
     };
     if (l == P->levels - 2) {
-        for (int i = 0; i < c.nu1 + c.nu2; ++i) smooth_ops(l);
+        for (int i = 0; i < c.nu1 + c.nu2; ++i) smooth_ops(l, red_done && i == 0);
         return;
     }
-    for (int i = 0; i < c.nu1; ++i) smooth_ops(l);
-    ops.push_back({jsv::CO_RESTRICT, l, 0, 1});      // also phi of l + 1 := 0
+    for (int i = 0; i < c.nu1; ++i) smooth_ops(l, red_done && i == 0);
+    const int first_sweeps = (l + 1 == P->levels - 2) ? c.nu1 + c.nu2 : c.nu1;
+    const bool red = fuse && first_sweeps >= 1;
+    ops.push_back({jsv::CO_RESTRICT, l, red ? 1 : 0, 1});      // also phi of l + 1 := 0
+    bool first = red;
     if (fcyc)
-        for (int i = 0; i < c.gamma; ++i) build_ops(P, c, l + 1, true, ops);
-    for (int i = 0; i < c.gamma; ++i) build_ops(P, c, l + 1, false, ops);
+        for (int i = 0; i < c.gamma; ++i) {
+            build_ops(P, G, c, l + 1, true, ops, first);
+            first = false;
+        }
+    for (int i = 0; i < c.gamma; ++i) {
+        build_ops(P, G, c, l + 1, false, ops, first);
+        first = false;
+    }
     const int full = (c.nu2 <= 0 || (P->dim == 2 && P->offdiag)) ? 1 : 0;
+    if (fuse && c.nu2 >= 1 && !op_empty(P, G, {jsv::CO_RB, l, first_par, 0})) {
+        ops.push_back({jsv::CO_PRO_RB, l, first_par, 0});
+        ops.push_back({jsv::CO_RB, l, 1 - first_par, 0});
+        for (int i = 1; i < c.nu2; ++i) smooth_ops(l, false);
+        return;
+    }
     ops.push_back({jsv::CO_PROLONG, l, full, 0});
-    for (int i = 0; i < c.nu2; ++i) smooth_ops(l);
+    for (int i = 0; i < c.nu2; ++i) smooth_ops(l, false);
 }
 
 template <typename T>
@@ -723,7 +748,8 @@ const jsv_plan::CoarseProg* find_prog(const jsv_plan* P, const jsv_config& c, bo
     for (const auto& pr : P->progs)
         if (pr.prec == prec && pr.gamma == c.gamma && pr.nu1 == c.nu1 && pr.nu2 == c.nu2 &&
             pr.fcyc == (fcyc ? 1 : 0) && pr.start == start && pr.offdiag == (P->offdiag ? 1 : 0) &&
-            pr.lds_from == P->clds_from[pi] && pr.solo == P->coarse_solo && pr.reg == P->coarse_reg)
+            pr.lds_from == P->clds_from[pi] && pr.solo == P->coarse_solo && pr.reg == P->coarse_reg &&
+            pr.fuse == P->fuse_halves)
             return &pr;
     return nullptr;
 }
@@ -757,7 +783,8 @@ long op_points(const jsv_plan* P, const std::vector<Geo>& G, const std::vector<i
     const long full = staged ? ln[op.level] : (P->dim == 3 ? g.n : g.sz);
     switch (op.kind) {
     case jsv::CO_RB:
-    case jsv::CO_RB_PAIR: return (long)(g.mx / 2) * g.my * nz;
+    case jsv::CO_RB_PAIR:
+    case jsv::CO_PRO_RB: return (long)(g.mx / 2) * g.my * nz;
     case jsv::CO_RB_DIAG: return (long)(g.mx / 2) * (g.my / 2);
     case jsv::CO_ZERO: return full;
     case jsv::CO_PROLONG: return (long)g.mx * g.my * nz;
@@ -804,8 +831,8 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         std::vector<jsv::CoarseOp>& ops = ops_of[fc];
         ops.push_back({jsv::CO_ZERO, start, 0, 0, 0});
         if (fc)
-            for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, true, ops);
-        for (int i = 0; i < c.gamma; ++i) build_ops(P, c, start, false, ops);
+            for (int i = 0; i < c.gamma; ++i) build_ops(P, H.g, c, start, true, ops);
+        for (int i = 0; i < c.gamma; ++i) build_ops(P, H.g, c, start, false, ops);
     }
     const size_t op_bytes = 4 * std::max(ops_of[0].size(), ops_of[1].size());
     size_t nar = 0;
@@ -895,16 +922,17 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         std::vector<jsv::CoarseOp>& ops = ops_of[fc];
         if (ops.size() > (size_t)jsv::COARSE_MAX_OPS) {
             P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
-                                P->offdiag ? 1 : 0, lds_from, P->coarse_solo, P->coarse_reg, nullptr,
-                                (int)ops.size()});
+                                P->offdiag ? 1 : 0, lds_from, P->coarse_solo, P->coarse_reg, P->fuse_halves,
+                                nullptr, (int)ops.size()});
             continue;
         }
         // Ops that change nothing go: smoothing halves without points, and on
         // a level whose only unknown neighbours boundary nodes every
         // smoothing op after the first of a visit (the unknown's update reads
-        // only rhs and coefficients, so it recomputes the same bits).  The
-        // first op zeroes a staged start level's phi, which the image
-        // already holds as zeros.
+        // only rhs and coefficients, so it recomputes the same bits; a
+        // restriction that runs the level's first half counts as that
+        // first).  The first op zeroes a staged start level's phi, which the
+        // image already holds as zeros.
         auto single = [&](int l) {
             const Geo& g = H.g[l];
             return g.mx == 2 && g.my == 2 && (P->dim == 2 || g.mz == 2);
@@ -918,8 +946,11 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
                 if (smooth && single(op.level)) {
                     if (done_single) continue;
                     done_single = true;
-                } else if (!(smooth && single(op.level))) {
-                    done_single = false;
+                } else {
+                    // the single unknown (2, 2[, 2]) has even parity: the first
+                    // half in 2D, which a restriction with a = 1 runs; in 3D it
+                    // is the second half's
+                    done_single = op.kind == jsv::CO_RESTRICT && op.a == 1 && single(op.level + 1) && P->dim == 2;
                 }
                 if (op.kind == jsv::CO_ZERO && op.level == start && start >= lds_from && start < lds_to &&
                     kept.empty())
@@ -931,7 +962,7 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         // runs of small ops on staged levels go to one wave
         auto staged = [&](int l) { return l >= lds_from && l < lds_to; };
         auto solo = [&](const jsv::CoarseOp& op) {
-            const bool two = op.kind == jsv::CO_RESTRICT || op.kind == jsv::CO_PROLONG;
+            const bool two = op.kind == jsv::CO_RESTRICT || op.kind == jsv::CO_PROLONG || op.kind == jsv::CO_PRO_RB;
             return P->coarse_solo > 0 && staged(op.level) && (!two || staged(op.level + 1)) &&
                    op_points(P, H.g, ln, true, op) <= P->coarse_solo;
         };
@@ -943,6 +974,10 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
                 if (i + 1 < ops.size() && a.kind == jsv::CO_RB && ops[i + 1].kind == jsv::CO_RB &&
                     ops[i + 1].level == a.level && ops[i + 1].a == 1 - a.a && solo(a) && solo(ops[i + 1])) {
                     merged.push_back({jsv::CO_RB_PAIR, a.level, a.a, 0, 0});
+                    ++i;
+                } else if (i + 1 < ops.size() && a.kind == jsv::CO_PRO_RB && ops[i + 1].kind == jsv::CO_RB &&
+                           ops[i + 1].level == a.level && ops[i + 1].a == 1 - a.a && solo(a) && solo(ops[i + 1])) {
+                    merged.push_back({jsv::CO_PRO_RB, a.level, a.a, 1, 0});
                     ++i;
                 } else {
                     merged.push_back(a);
@@ -959,7 +994,9 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
             for (size_t j = i; j < k; ++j) {
                 const jsv::CoarseOp& q = ops[j];
                 if (q.kind == jsv::CO_RB_DIAG) return false;
-                const int hi = (q.kind == jsv::CO_RESTRICT || q.kind == jsv::CO_PROLONG) ? q.level + 1 : q.level;
+                const int hi = (q.kind == jsv::CO_RESTRICT || q.kind == jsv::CO_PROLONG || q.kind == jsv::CO_PRO_RB)
+                                   ? q.level + 1
+                                   : q.level;
                 lmin = std::min(lmin, q.level);
                 lmax = std::max(lmax, hi);
             }
@@ -985,7 +1022,8 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
         RETCHK(dalloc(P, (void**)&d, sizeof(unsigned) * words.size()));
         HIPCHK(hipMemcpy(d, words.data(), sizeof(unsigned) * words.size(), hipMemcpyHostToDevice));
         P->progs.push_back({sizeof(T) == 4 ? JSV_MIXED : JSV_FP64, c.gamma, c.nu1, c.nu2, fc, start,
-                            P->offdiag ? 1 : 0, lds_from, P->coarse_solo, P->coarse_reg, d, (int)ops.size()});
+                            P->offdiag ? 1 : 0, lds_from, P->coarse_solo, P->coarse_reg, P->fuse_halves, d,
+                            (int)ops.size()});
     }
     return 0;
 }

@@ -1365,10 +1365,13 @@ struct LvPtr {
 enum CoarseOpKind : int {
     CO_RB = 0,        // one RBGS half: a = parity
     CO_RB_DIAG = 1,   // one D_xy sub-colour: a = lx, b = ly
-    CO_RESTRICT = 2,  // defect of `level` restricted into rhs of level + 1; b = 1: also phi of level + 1 := 0
+    CO_RESTRICT = 2,  // defect of `level` restricted into rhs of level + 1; b = 1: also phi of level + 1 := 0;
+                      // a = 1 (2D, with b = 1): level + 1's first RBGS half from that zero phi as well
     CO_ZERO = 3,      // phi of `level` := 0 (whole array incl. rim)
     CO_PROLONG = 4,   // phi of `level` += P phi of level + 1; a = full
     CO_RB_PAIR = 5,   // both RBGS halves, parity a first (one-wave runs only)
+    CO_PRO_RB = 6,    // 2D: the prolongation (full = 0) inside the RBGS half of parity a; b = 1: then
+                      // the other half (one-wave runs only)
 };
 
 struct CoarseOp {
@@ -1536,7 +1539,7 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
     const LvView<T, FL> F(LF, sm, (dim == 2 && !FL) ? (long)b * g.sz : 0, dim);
     // loop extents are the resolutions: padded past the last unknown to a power of two
     auto lg = [](int v) { return v > 0 ? __builtin_ctz((unsigned)v) : 0; };
-    if (op.kind == CO_RB || op.kind == CO_RB_PAIR) {
+    if (op.kind == CO_RB || op.kind == CO_RB_PAIR || op.kind == CO_PRO_RB) {
         const int xh = g.mx / 2, ny = g.my, lxh = lg(xh), lny = lg(ny);
         auto half = [&](int par) {
             if constexpr (dim == 3) {
@@ -1560,8 +1563,44 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
                 }
             }
         };
-        half(op.a);
-        if (op.kind == CO_RB_PAIR) {
+        if (op.kind == CO_PRO_RB) {
+            // 2D only (the program builder fuses in 2D without D_xy): the
+            // prolongation inside the half of parity a, k_rb2_pro's arithmetic.
+            // This half reads only the other parity, each value taken as stored
+            // value plus its correction, and nothing reads the other parity's
+            // corrected values later (its half overwrites them without reading
+            // them).
+            if constexpr (dim == 2) {
+                const LvPtr<T>& LC = lv[op.level + 1];
+                const Geo gc = LC.g;
+                const LvView<T, CL> C(LC, sm, !CL ? (long)b * gc.sz : 0, dim);
+                const int par = op.a;
+                auto C2 = [&](int J, int K) -> T { return C.phi[C.at(J, K)]; };
+                auto nbv = [&](int xx, int yy, long q) -> T {
+                    T v = F.phi[q];
+                    T corr;
+                    if (xx >= 2 && xx <= g.mx && yy >= 2 && yy <= g.my && prolong2_corr_at<T>(C2, xx, yy, 0, corr))
+                        v = v + corr;
+                    return v;
+                };
+                const int cnt = xh * ny;
+                for (int i = t0; i < cnt; i += nt) {
+                    const Ix2 yx = split2(i, xh, lxh);
+                    const int y = 2 + yx.hi;
+                    const int x = 2 + ((par + y) & 1) + 2 * yx.lo;
+                    if (x > g.mx || y > g.my) continue;
+                    const long p = F.at(y, x);
+                    T acc = F.cy[p] * nbv(x, y + 1, p + F.sy) + F.cy[p - F.sy] * nbv(x, y - 1, p - F.sy);
+                    acc = acc + F.cx[p] * nbv(x + 1, y, p + 1);
+                    acc = acc + F.cx[p - 1] * nbv(x - 1, y, p - 1);
+                    acc = acc - F.rhs[p];
+                    F.phi[p] = acc / F.cc[p];
+                }
+            }
+        } else {
+            half(op.a);
+        }
+        if (op.kind == CO_RB_PAIR || (op.kind == CO_PRO_RB && op.b)) {
             // one wave (runs only): the second half reads what other lanes wrote
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1637,10 +1676,15 @@ __device__ __forceinline__ void coarse_op(const CoarseOp& op, const LvPtr<T>* __
                 const long c = F.at(2 * J - 1, 2 * K - 1);
                 const T v = F.od ? restrict2_at(F.phi, F.rhs, F.cx, F.cy, F.cc, F.drr, F.dlr, c, F.sy)
                                  : restrict2_at_pre(F.phi, F.rhs, F.cx, F.cy, F.cc, c, F.sy);
-                C.rhs[C.at(J, K)] = v;
+                const long q = C.at(J, K);
+                C.rhs[q] = v;
+                if (op.a)
+                    C.phi[q] = ((J + K) & 1) == 0 ? rb2_from_zero(v, RedZ<T>{C.cx, C.cy, nullptr, C.cc}, q, C.sy) : (T)0;
             }
         }
-        if (op.b)
+        // with a, every unknown of level + 1 was written above; its rim is
+        // never written by any op and holds zeros
+        if (op.b && !op.a)
             for (int i = t0; i < C.zn; i += nt) C.phi[i] = (T)0;
     } else if (op.kind == CO_ZERO) {
         for (int i = t0; i < F.zn; i += nt) F.phi[i] = (T)0;
@@ -1725,27 +1769,34 @@ __device__ __forceinline__ void reg_store(const RegLv<T>& S, const LvPtr<T>& L, 
     }
 }
 
-// phi at the four neighbours of this lane's point; 0 past the low boundary
-// (index 1), the high boundary lanes hold 0
+// v at the four neighbours of this lane's point (v: the level's phi, or phi
+// with the prolongation's corrections); 0 past the low boundary (index 1),
+// the high boundary lanes hold 0
 template <typename T>
-__device__ __forceinline__ void reg_nbrs(const RegLv<T>& S, int lane, T& pxp, T& pxm, T& pyp, T& pym)
+__device__ __forceinline__ void reg_nbrs(const RegLv<T>& S, T v, int lane, T& pxp, T& pxm, T& pyp, T& pym)
 {
     const int xi = lane & (S.mx - 1), yi = lane >> S.lmx;
-    pxp = __shfl(S.phi, (lane + 1) & 63, 64);
-    pxm = __shfl(S.phi, (lane - 1) & 63, 64);
-    pyp = __shfl(S.phi, (lane + S.mx) & 63, 64);
-    pym = __shfl(S.phi, (lane - S.mx) & 63, 64);
+    pxp = __shfl(v, (lane + 1) & 63, 64);
+    pxm = __shfl(v, (lane - 1) & 63, 64);
+    pyp = __shfl(v, (lane + S.mx) & 63, 64);
+    pym = __shfl(v, (lane - S.mx) & 63, 64);
     if (xi == 0) pxm = (T)0;
     if (yi == 0) pym = (T)0;
 }
-
 template <typename T>
-__device__ __forceinline__ void reg_rb(RegLv<T>& S, int par, int lane)
+__device__ __forceinline__ void reg_nbrs(const RegLv<T>& S, int lane, T& pxp, T& pxm, T& pyp, T& pym)
+{
+    reg_nbrs(S, S.phi, lane, pxp, pxm, pyp, pym);
+}
+
+// one RBGS half of parity par; nb: the values its neighbours are read from
+template <typename T>
+__device__ __forceinline__ void reg_rb_from(RegLv<T>& S, T nb, int par, int lane)
 {
     const int xi = lane & (S.mx - 1), yi = lane >> S.lmx;
     const bool upd = lane < S.mx * S.my && xi <= S.mx - 2 && yi <= S.my - 2 && ((xi + yi) & 1) == par;
     T pxp, pxm, pyp, pym;
-    reg_nbrs(S, lane, pxp, pxm, pyp, pym);
+    reg_nbrs(S, nb, lane, pxp, pxm, pyp, pym);
     T acc = S.cyp * pyp + S.cym * pym;
     acc = acc + S.cxp * pxp;
     acc = acc + S.cxm * pxm;
@@ -1755,7 +1806,14 @@ __device__ __forceinline__ void reg_rb(RegLv<T>& S, int par, int lane)
 }
 
 template <typename T>
-__device__ __forceinline__ void reg_restrict(const RegLv<T>& F, RegLv<T>& C, int zero, int lane)
+__device__ __forceinline__ void reg_rb(RegLv<T>& S, int par, int lane)
+{
+    reg_rb_from(S, S.phi, par, lane);
+}
+
+// red: the coarse level's first half from phi = 0 as well (rb2_from_zero)
+template <typename T>
+__device__ __forceinline__ void reg_restrict(const RegLv<T>& F, RegLv<T>& C, int zero, int red, int lane)
 {
     T pxp, pxm, pyp, pym;
     reg_nbrs(F, lane, pxp, pxm, pyp, pym);
@@ -1777,11 +1835,22 @@ __device__ __forceinline__ void reg_restrict(const RegLv<T>& F, RegLv<T>& C, int
     const T v = (T)0.0625 * sum + (T)0.25 * rc;
     const bool cunk = lane < C.mx * C.my && xc <= C.mx - 2 && yc <= C.my - 2;
     C.rhs = cunk ? v : C.rhs;
-    C.phi = zero ? (T)0 : C.phi;
+    if (red) {
+        const T o = (T)0;
+        T acc = C.cyp * o + C.cym * o;
+        acc = acc + C.cxp * o;
+        acc = acc + C.cxm * o;
+        acc = acc - v;
+        const T pv = acc / C.cc;
+        C.phi = (cunk && ((xc + yc) & 1) == 0) ? pv : (T)0;
+    } else {
+        C.phi = zero ? (T)0 : C.phi;
+    }
 }
 
+// the prolongation's correction of this lane's fine point; false: none
 template <typename T>
-__device__ __forceinline__ void reg_prolong(RegLv<T>& F, const RegLv<T>& C, int full, int lane)
+__device__ __forceinline__ bool reg_corr(const RegLv<T>& F, const RegLv<T>& C, int full, int lane, T& corr)
 {
     const int xi = lane & (F.mx - 1), yi = lane >> F.lmx;
     const int x = 2 + xi, y = 2 + yi;
@@ -1800,8 +1869,29 @@ __device__ __forceinline__ void reg_prolong(RegLv<T>& F, const RegLv<T>& C, int 
     sum = sum + Cc;
     sum = sum + D;
     const T c_e = (T)0.25 * sum;
-    const T corr = (ox && oy) ? A : oy ? c_y : ox ? c_x : c_e;
-    const bool upd = lane < F.mx * F.my && xi <= F.mx - 2 && yi <= F.my - 2 && (ox || oy || full);
+    corr = (ox && oy) ? A : oy ? c_y : ox ? c_x : c_e;
+    return lane < F.mx * F.my && xi <= F.mx - 2 && yi <= F.my - 2 && (ox || oy || full);
+}
+
+// the prolongation (full = 0) inside the half of parity par, then (pair) the
+// other half: the half reads its neighbours as phi plus their corrections
+// (reg_prolong's values), and the other parity keeps its uncorrected phi,
+// which its own half overwrites without reading
+template <typename T>
+__device__ __forceinline__ void reg_pro_rb(RegLv<T>& F, const RegLv<T>& C, int par, int pair, int lane)
+{
+    T corr;
+    const bool has = reg_corr(F, C, 0, lane, corr);
+    const T nb = has ? F.phi + corr : F.phi;
+    reg_rb_from(F, nb, par, lane);
+    if (pair) reg_rb(F, 1 - par, lane);
+}
+
+template <typename T>
+__device__ __forceinline__ void reg_prolong(RegLv<T>& F, const RegLv<T>& C, int full, int lane)
+{
+    T corr;
+    const bool upd = reg_corr(F, C, full, lane, corr);
     F.phi = upd ? F.phi + corr : F.phi;
 }
 
@@ -1815,7 +1905,7 @@ __device__ __forceinline__ void coarse_reg_run(const unsigned* sops, int o, int 
     int lmin = 31, lmax = 0;
     for (int k = 0; k < run; ++k) {
         const CoarseOp q = coarse_unpack(__builtin_amdgcn_readfirstlane(sops[o + k]));
-        const int hi = (q.kind == CO_RESTRICT || q.kind == CO_PROLONG) ? q.level + 1 : q.level;
+        const int hi = (q.kind == CO_RESTRICT || q.kind == CO_PROLONG || q.kind == CO_PRO_RB) ? q.level + 1 : q.level;
         lmin = q.level < lmin ? q.level : lmin;
         lmax = hi > lmax ? hi : lmax;
     }
@@ -1837,8 +1927,11 @@ __device__ __forceinline__ void coarse_reg_run(const unsigned* sops, int o, int 
             else if (j == 1) sweep(S1);
             else sweep(S2);
         } else if (q.kind == CO_RESTRICT) {
-            if (j == 0) reg_restrict(S0, S1, q.b, lane);
-            else reg_restrict(S1, S2, q.b, lane);
+            if (j == 0) reg_restrict(S0, S1, q.b, q.a, lane);
+            else reg_restrict(S1, S2, q.b, q.a, lane);
+        } else if (q.kind == CO_PRO_RB) {
+            if (j == 0) reg_pro_rb(S0, S1, q.a, q.b, lane);
+            else reg_pro_rb(S1, S2, q.a, q.b, lane);
         } else if (q.kind == CO_PROLONG) {
             if (j == 0) reg_prolong(S0, S1, q.a, lane);
             else reg_prolong(S1, S2, q.a, lane);

@@ -264,6 +264,10 @@ CYCLES = [
     ("V_nu2_0", dict(nu1=2, nu2=0)),
     ("W_nu1_2_nu2_3", dict(gamma=2, nu1=2, nu2=3)),
 ]
+# the coarse program's fused ops change with nu1 and nu2 (no first half in the
+# restriction when nu1 = 0 above the coarsest level)
+FUSED_PROG_CYCLES = [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4], CYCLES[5],
+                     ("W_nu1_0_nu2_2", dict(gamma=2, nu1=0, nu2=2))]
 
 
 @pytest.mark.parametrize("cyc", CYCLES, ids=[c[0] for c in CYCLES])
@@ -365,15 +369,19 @@ def test_cycle_fused_halves_fp32_matches_separate(dim, ms):
     assert np.abs(outs[0]).max() > 0
 
 
+@pytest.mark.parametrize("fuse", [1, 0])
 @pytest.mark.parametrize("solo,threads", [(0, 1024), (64, 256), (64, 512), (16, 1024)])
-@pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4]], ids=lambda c: c[0])
+@pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4], FUSED_PROG_CYCLES[5]],
+                         ids=lambda c: c[0])
 @pytest.mark.parametrize("case", COARSE_CASES, ids=[c[0] for c in COARSE_CASES])
-def test_cycle_coarse_program_launch_shapes(case, cyc, solo, threads):
+def test_cycle_coarse_program_launch_shapes(case, cyc, solo, threads, fuse):
     """The coarse program from level 1 down with its small ops run by one wave
-    (or never) and other workgroup sizes: still the oracle's bits."""
+    (or never) and other workgroup sizes, its fused ops on or off: still the
+    oracle's bits."""
     from jsolver_amd import _lib
     lib = _lib.load()
     dp, op, phi, rhs = _setup(case, seed=19)
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_HALVES, fuse))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, 1 << 20))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_SOLO, solo))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_THREADS, threads))
@@ -391,17 +399,22 @@ REG_CASES = [c for c in CASES if c[0] in ("2d_spatial_16x32", "2d_spatial_32x8",
     ("2d_spatial_128x64", 2, (128, 64), True, True, False)]
 
 
+@pytest.mark.parametrize("fuse", [1, 0])
 @pytest.mark.parametrize("reg", [1, 0])
-@pytest.mark.parametrize("cyc", [CYCLES[0], CYCLES[1], CYCLES[2], CYCLES[4], CYCLES[5]], ids=lambda c: c[0])
+@pytest.mark.parametrize("cyc", FUSED_PROG_CYCLES, ids=lambda c: c[0])
 @pytest.mark.parametrize("case", REG_CASES, ids=[c[0] for c in REG_CASES])
-def test_cycle_coarse_program_register_runs(case, cyc, reg):
+def test_cycle_coarse_program_register_runs(case, cyc, reg, fuse):
     """2D one-wave runs with their levels in registers (neighbours by lane
-    shuffles; square and rectangular levels) and in LDS: the oracle's bits."""
+    shuffles; square and rectangular levels) and in LDS, with the restriction
+    running the coarse level's first half and the prolongation inside the first
+    post-smoothing half (JSV_TUNE_FUSE_HALVES) or as separate ops: the
+    oracle's bits."""
     from jsolver_amd import _lib
     lib = _lib.load()
     dp, op, phi, rhs = _setup(case, seed=29)
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_M, 1 << 20))
     _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_REG, reg))
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSE_HALVES, fuse))
     kw = cyc[1]
     c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
     want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
