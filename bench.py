@@ -68,7 +68,7 @@ def roofline(a, grids, m):
             _lib.check(lib.jsv_smooth(plan.h, prec, level, 2, stream_ptr()))
             # median over blocks of `sweeps` back-to-back sweeps
             blocks = []
-            for _ in range(a.sweep_blocks):
+            for _ in range(max(1, a.sweep_blocks)):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
                 _lib.check(lib.jsv_smooth(plan.h, prec, level, sweeps, stream_ptr()))
