@@ -132,8 +132,9 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *   JSV_TUNE_COARSE_SOLO  ops of that launch on LDS levels touching at most
  *                         this many points run in one wave without workgroup
  *                         barriers (default 64, at most 64; 0 = never)
- *   JSV_TUNE_COARSE_THREADS workgroup size of that launch: 256, 512 or 1024
- *                         (default 1024)
+ *   JSV_TUNE_COARSE_THREADS workgroup size of that launch: 256, 512 or 1024;
+ *                         0 (default): 512 for 2D FP64 programs, 1024 for
+ *                         the others
  *   JSV_TUNE_FUSE_HALVES  levels smoothed with one launch per RBGS half (2D
  *                         without D_xy; 3D below JSV_TUNE_FUSED_MIN_M): 1
  *                         (default) the restriction into such a level also
@@ -150,10 +151,13 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         a relative residual summed inside the first sweep
  *                         of a cycle may differ from the host's sum in the
  *                         last bits
- *   JSV_TUNE_COARSE_REG   1 (default): in 2D without D_xy, the coarse
+ *   JSV_TUNE_COARSE_REG   2 (default): in 2D without D_xy, the coarse
  *                         program's one-wave runs on levels of at most 64
  *                         points keep those levels in registers (neighbours
- *                         by lane shuffles); 0: in LDS
+ *                         by lane shuffles), and so do runs from a level of
+ *                         at most 256 points (a 2 x 2 block per lane) down
+ *                         through at most three levels below it; 1: the
+ *                         former only; 0: in LDS
  *   JSV_TUNE_ZIGZAG       1 (default): a pair sweep without the prolongation
  *                         or the residual sum marches its level's planes in
  *                         the opposite order of that level's previous pair

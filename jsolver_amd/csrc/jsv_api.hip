@@ -133,8 +133,9 @@ struct jsv_plan {
     std::vector<const void*> lv_host[2];           // pointers lv_dev holds (start level + table)
     int coarse_lds = 1;             // stage the program's coarsest levels in LDS
     int coarse_solo = jsv::COARSE_WAVE;   // ops on staged levels with at most this many points run in one wave
-    int coarse_reg = 1;             // 2D runs of such ops keep their levels' points in registers
-    int coarse_threads = jsv::COARSE_THREADS;   // workgroup size of the coarse program
+    int coarse_reg = 2;             // 2D runs of such ops keep their levels' points in registers (2: and
+                                    // runs from a level of up to 256 points, four per lane)
+    int coarse_threads = 0;   // workgroup size of the coarse program; 0: 512 for 2D FP64, else 1024
     int clds_from[2] = {0, 0}, clds_to[2] = {0, 0};   // staged level range per precision
     size_t clds_bytes[2] = {0, 0};
     void* img_dev[2] = {nullptr, nullptr};   // coefficient image of the staged levels (per problem)
@@ -986,8 +987,11 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
             ops.swap(merged);
         }
         // a run keeps its levels' points in registers when it touches at most
-        // three consecutive 2D levels of at most 64 points (padded), and no
-        // D_xy sub-colours
+        // three consecutive 2D levels of at most 64 points (padded), or
+        // (coarse_reg 2) a level of at most 256 points with even extents and
+        // up to three such levels below it, each half the one above per axis;
+        // and no D_xy sub-colours
+        auto wave_pts = [&](int l) { return (long)H.g[l].mx * H.g[l].my; };
         auto reg_ok = [&](size_t i, size_t k) {
             if (!P->coarse_reg || P->dim != 2 || P->offdiag) return false;
             int lmin = 1 << 30, lmax = -1;
@@ -1000,14 +1004,36 @@ int prepare_coarse(jsv_plan* P, Hier<T>& H, const jsv_config& c, hipStream_t s)
                 lmin = std::min(lmin, q.level);
                 lmax = std::max(lmax, hi);
             }
-            if (lmax - lmin > 2) return false;
+            const bool four = wave_pts(lmin) > jsv::COARSE_WAVE;
+            if (four) {
+                const Geo& g = H.g[lmin];
+                if (P->coarse_reg < 2 || wave_pts(lmin) > 4 * jsv::COARSE_WAVE || (g.mx & 1) || (g.my & 1) ||
+                    lmax - lmin > 3)
+                    return false;
+                for (int l = lmin + 1; l <= lmax; ++l)
+                    if (2 * H.g[l].mx != H.g[l - 1].mx || 2 * H.g[l].my != H.g[l - 1].my) return false;
+            } else if (lmax - lmin > 2) {
+                return false;
+            }
             for (int l = lmin; l <= lmax; ++l)
-                if (!staged(l) || (long)H.g[l].mx * H.g[l].my > jsv::COARSE_WAVE) return false;
+                if (!staged(l) || ((l > lmin || !four) && wave_pts(l) > jsv::COARSE_WAVE)) return false;
             return true;
+        };
+        // an op of such a run that one wave would not run alone otherwise
+        auto solo4 = [&](const jsv::CoarseOp& op) {
+            const bool two = op.kind == jsv::CO_RESTRICT || op.kind == jsv::CO_PROLONG || op.kind == jsv::CO_PRO_RB;
+            return P->coarse_solo > 0 && P->coarse_reg >= 2 && P->dim == 2 && !P->offdiag &&
+                   op.kind != jsv::CO_RB_DIAG &&
+                   staged(op.level) && (!two || staged(op.level + 1)) &&
+                   wave_pts(op.level) <= 4 * jsv::COARSE_WAVE;
         };
         for (size_t i = 0; i < ops.size();) {
             size_t k = i;
             while (k < ops.size() && solo(ops[k])) ++k;
+            // a longer run that includes ops on a level of up to 256 points
+            size_t k4 = i;
+            while (k4 < ops.size() && (solo(ops[k4]) || solo4(ops[k4]))) ++k4;
+            if (k4 > k && reg_ok(i, k4)) k = k4;
             if (k > i) {
                 ops[i].run = (int)(k - i);
                 ops[i].reg = reg_ok(i, k) ? 1 : 0;
@@ -1119,7 +1145,11 @@ int mg_cycle_body(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, boo
             hipLaunchKernelGGL(kern, grid, dim3(nt), lds, s, lvp, pr->ops, pr->nops, ops_at, P->act_ptr(), lf, lt,
                                wb, img, imn, (long)imn);
         };
-        const int nt = P->coarse_threads;
+        // 2D FP64 register runs from 256-point levels need more than the 128
+        // VGPRs a 1024-thread workgroup leaves a wave
+        const int nt = P->coarse_threads != 0                  ? P->coarse_threads
+                       : (P->dim == 2 && sizeof(T) == 8) ? 512
+                                                         : jsv::COARSE_THREADS;
         if (P->dim == 2)
             nt == 256   ? launch(jsv::k_coarse_prog<T, 256, 2>, nt)
             : nt == 512 ? launch(jsv::k_coarse_prog<T, 512, 2>, nt)
@@ -1537,7 +1567,8 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         return 0;
     }
     if (key == JSV_TUNE_COARSE_THREADS) {
-        if (value != 256 && value != 512 && value != 1024) return fail("coarse program threads must be 256, 512 or 1024");
+        if (value != 0 && value != 256 && value != 512 && value != 1024)
+            return fail("coarse program threads must be 0, 256, 512 or 1024");
         P->coarse_threads = (int)value;
         return 0;
     }
@@ -1558,7 +1589,8 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         return 0;
     }
     if (key == JSV_TUNE_COARSE_REG) {
-        P->coarse_reg = value != 0 ? 1 : 0;
+        if (value > 2) return fail("coarse reg must be 0, 1 or 2");
+        P->coarse_reg = (int)value;
         return 0;
     }
     if (key == JSV_TUNE_FUSE_PROLONG) {

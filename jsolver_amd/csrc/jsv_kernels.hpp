@@ -1895,9 +1895,290 @@ __device__ __forceinline__ void reg_prolong(RegLv<T>& F, const RegLv<T>& C, int 
     F.phi = upd ? F.phi + corr : F.phi;
 }
 
+// 2D levels of 65 to 256 points (padded indices 2..m+1) in a register run:
+// lane (by, bx) = by * (mx / 2) + bx holds the 2 x 2 block of points
+// (2 + 2by + dy, 2 + 2bx + dx), element k = 2 dy + dx.  The next coarser
+// level's one-point layout (lane = yc * (mx / 2) + xc) puts coarse point
+// (2 + yc, 2 + xc) in the lane of the block that holds its restriction centre
+// (element 3) and one of its corners (element 0), so the restriction and the
+// prolongation between the two levels need only three shuffles.
+template <typename T>
+struct RegLv4 {
+    T phi[4], rhs[4], cxp[4], cyp[4], cc[4];
+    // cx[p - 1] of elements 0 and 2, cy[p - sy] of elements 0 and 1; those of
+    // the other elements are the lane's own cxp[0], cxp[2] and cyp[0], cyp[1]
+    T cxm0, cxm2, cym0, cym1;
+    int w, h, lw;   // blocks per row (mx / 2), block rows (my / 2), log2(w)
+};
+
+template <typename T>
+__device__ __forceinline__ void reg4_load(RegLv4<T>& Q, const LvPtr<T>& L, T* sm, int lane)
+{
+    Q.w = L.g.mx >> 1;
+    Q.h = L.g.my >> 1;
+    Q.lw = __builtin_ctz((unsigned)Q.w);
+    const LvView<T, true> V(L, sm, 0, 2);
+    const bool in = lane < Q.w * Q.h;
+    const int bx = lane & (Q.w - 1), by = lane >> Q.lw;
+    const int y0 = in ? 2 + 2 * by : 2, x0 = in ? 2 + 2 * bx : 2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long p = V.at(y0 + (k >> 1), x0 + (k & 1));
+        const T a = V.phi[p];
+        Q.phi[k] = in ? a : (T)0;
+        Q.rhs[k] = V.rhs[p];
+        Q.cxp[k] = V.cx[p];
+        Q.cyp[k] = V.cy[p];
+        Q.cc[k] = V.cc[p];
+    }
+    const long p0 = V.at(y0, x0), p1 = V.at(y0, x0 + 1), p2 = V.at(y0 + 1, x0);
+    Q.cxm0 = V.cx[p0 - 1];
+    Q.cxm2 = V.cx[p2 - 1];
+    Q.cym0 = V.cy[p0 - V.sy];
+    Q.cym1 = V.cy[p1 - V.sy];
+}
+
+template <typename T>
+__device__ __forceinline__ void reg4_store(const RegLv4<T>& Q, const LvPtr<T>& L, T* sm, int lane)
+{
+    const LvView<T, true> V(L, sm, 0, 2);
+    if (lane < Q.w * Q.h) {
+        const int bx = lane & (Q.w - 1), by = lane >> Q.lw;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long p = V.at(2 + 2 * by + (k >> 1), 2 + 2 * bx + (k & 1));
+            V.phi[p] = Q.phi[k];
+            V.rhs[p] = Q.rhs[k];
+        }
+    }
+}
+
+// element k of this lane's block is an unknown (not the boundary row / column)
+template <typename T>
+__device__ __forceinline__ bool reg4_unk(const RegLv4<T>& Q, int lane, int k)
+{
+    const int bx = lane & (Q.w - 1), by = lane >> Q.lw;
+    return lane < Q.w * Q.h && 2 * bx + (k & 1) <= 2 * Q.w - 2 && 2 * by + (k >> 1) <= 2 * Q.h - 2;
+}
+
+// one RBGS half of parity par (elements 0 and 3 for par 0, 1 and 2 for par
+// 1), its neighbours read from n0..n3 (phi, or phi plus the prolongation's
+// corrections); rb2_point_pre's arithmetic in its order
+template <typename T>
+__device__ __forceinline__ void reg4_rb_from(RegLv4<T>& Q, T n0, T n1, T n2, T n3, int par, int lane)
+{
+    const int bx = lane & (Q.w - 1), by = lane >> Q.lw;
+    auto sh = [&](T v, int src) { return __shfl(v, src & 63, 64); };
+    if (par == 0) {
+        T xm0 = sh(n1, lane - 1), ym0 = sh(n2, lane - Q.w);
+        const T xp3 = sh(n2, lane + 1), yp3 = sh(n1, lane + Q.w);
+        if (bx == 0) xm0 = (T)0;
+        if (by == 0) ym0 = (T)0;
+        T a0 = Q.cyp[0] * n2 + Q.cym0 * ym0;
+        a0 = a0 + Q.cxp[0] * n1;
+        a0 = a0 + Q.cxm0 * xm0;
+        a0 = a0 - Q.rhs[0];
+        T a3 = Q.cyp[3] * yp3 + Q.cyp[1] * n1;
+        a3 = a3 + Q.cxp[3] * xp3;
+        a3 = a3 + Q.cxp[2] * n2;
+        a3 = a3 - Q.rhs[3];
+        const T v0 = a0 / Q.cc[0], v3 = a3 / Q.cc[3];
+        Q.phi[0] = reg4_unk(Q, lane, 0) ? v0 : Q.phi[0];
+        Q.phi[3] = reg4_unk(Q, lane, 3) ? v3 : Q.phi[3];
+    } else {
+        T ym1 = sh(n3, lane - Q.w), xm2 = sh(n3, lane - 1);
+        const T xp1 = sh(n0, lane + 1), yp2 = sh(n0, lane + Q.w);
+        if (by == 0) ym1 = (T)0;
+        if (bx == 0) xm2 = (T)0;
+        T a1 = Q.cyp[1] * n3 + Q.cym1 * ym1;
+        a1 = a1 + Q.cxp[1] * xp1;
+        a1 = a1 + Q.cxp[0] * n0;
+        a1 = a1 - Q.rhs[1];
+        T a2 = Q.cyp[2] * yp2 + Q.cyp[0] * n0;
+        a2 = a2 + Q.cxp[2] * n3;
+        a2 = a2 + Q.cxm2 * xm2;
+        a2 = a2 - Q.rhs[2];
+        const T v1 = a1 / Q.cc[1], v2 = a2 / Q.cc[2];
+        Q.phi[1] = reg4_unk(Q, lane, 1) ? v1 : Q.phi[1];
+        Q.phi[2] = reg4_unk(Q, lane, 2) ? v2 : Q.phi[2];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void reg4_rb(RegLv4<T>& Q, int par, int lane)
+{
+    reg4_rb_from(Q, Q.phi[0], Q.phi[1], Q.phi[2], Q.phi[3], par, lane);
+}
+
+// restriction of the 4-point level Q into the one-point level C
+// (reg_restrict's arithmetic: residuals in resid2's order at the corner
+// element 0 and the centre element 3, weighted in restrict2_at_pre's order)
+template <typename T>
+__device__ __forceinline__ void reg4_restrict(const RegLv4<T>& Q, RegLv<T>& C, int zero, int red, int lane)
+{
+    const int bx = lane & (Q.w - 1), by = lane >> Q.lw;
+    auto sh = [&](T v, int src) { return __shfl(v, src & 63, 64); };
+    T xm0 = sh(Q.phi[1], lane - 1), ym0 = sh(Q.phi[2], lane - Q.w);
+    const T xp3 = sh(Q.phi[2], lane + 1), yp3 = sh(Q.phi[1], lane + Q.w);
+    if (bx == 0) xm0 = (T)0;
+    if (by == 0) ym0 = (T)0;
+    T r0 = Q.cc[0] * Q.phi[0];
+    r0 = r0 - Q.cxm0 * xm0;
+    r0 = r0 - Q.cxp[0] * Q.phi[1];
+    r0 = r0 - Q.cym0 * ym0;
+    r0 = r0 - Q.cyp[0] * Q.phi[2];
+    r0 = r0 + Q.rhs[0];
+    T r3 = Q.cc[3] * Q.phi[3];
+    r3 = r3 - Q.cxp[2] * Q.phi[2];
+    r3 = r3 - Q.cxp[3] * xp3;
+    r3 = r3 - Q.cyp[1] * Q.phi[1];
+    r3 = r3 - Q.cyp[3] * yp3;
+    r3 = r3 + Q.rhs[3];
+    // coarse point (2 + by, 2 + bx): corners (y + 1, x + 1), (y + 1, x - 1),
+    // (y - 1, x + 1), (y - 1, x - 1) of its centre are element 0 of the blocks
+    // (by + 1, bx + 1), (by + 1, bx), (by, bx + 1) and this one
+    const T ra = sh(r0, lane + Q.w + 1), rb = sh(r0, lane + Q.w), rd = sh(r0, lane + 1);
+    T sum = ra + rb;
+    sum = sum + rd;
+    sum = sum + r0;
+    const T v = (T)0.0625 * sum + (T)0.25 * r3;
+    const bool cunk = lane < C.mx * C.my && bx <= C.mx - 2 && by <= C.my - 2;
+    C.rhs = cunk ? v : C.rhs;
+    if (red) {
+        const T o = (T)0;
+        T acc = C.cyp * o + C.cym * o;
+        acc = acc + C.cxp * o;
+        acc = acc + C.cxm * o;
+        acc = acc - v;
+        const T pv = acc / C.cc;
+        C.phi = (cunk && ((bx + by) & 1) == 0) ? pv : (T)0;
+    } else {
+        C.phi = zero ? (T)0 : C.phi;
+    }
+}
+
+// the prolongation's corrections of the block's elements from the one-point
+// level C (reg_corr's values); has[k]: element k takes its correction
+template <typename T>
+__device__ __forceinline__ void reg4_corr(const RegLv4<T>& Q, const RegLv<T>& C, int full, int lane, T (&corr)[4],
+                                          bool (&has)[4])
+{
+    const int bx = lane & (Q.w - 1), by = lane >> Q.lw;
+    auto sh = [&](T v, int src) { return __shfl(v, src & 63, 64); };
+    // coarse (2 + by, 2 + bx), (1 + by, 2 + bx), (2 + by, 1 + bx), (1 + by, 1 + bx); index 1 is the boundary
+    const T A = C.phi;
+    T B = sh(C.phi, lane - Q.w), Cc = sh(C.phi, lane - 1), D = sh(C.phi, lane - Q.w - 1);
+    if (by == 0) {
+        B = (T)0;
+        D = (T)0;
+    }
+    if (bx == 0) {
+        Cc = (T)0;
+        D = (T)0;
+    }
+    const T c_y = (T)0.5 * (A + Cc), c_x = (T)0.5 * (A + B);
+    T sum = A + B;
+    sum = sum + Cc;
+    sum = sum + D;
+    corr[0] = (T)0.25 * sum;   // (even, even)
+    corr[1] = c_x;             // x odd
+    corr[2] = c_y;             // y odd
+    corr[3] = A;               // (odd, odd)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) has[k] = reg4_unk(Q, lane, k) && (k != 0 || full);
+}
+
+template <typename T>
+__device__ __forceinline__ void reg4_prolong(RegLv4<T>& Q, const RegLv<T>& C, int full, int lane)
+{
+    T corr[4];
+    bool has[4];
+    reg4_corr(Q, C, full, lane, corr, has);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Q.phi[k] = has[k] ? Q.phi[k] + corr[k] : Q.phi[k];
+}
+
+// the prolongation (full = 0) inside the half of parity par, then (pair) the
+// other half, as reg_pro_rb
+template <typename T>
+__device__ __forceinline__ void reg4_pro_rb(RegLv4<T>& Q, const RegLv<T>& C, int par, int pair, int lane)
+{
+    T corr[4];
+    bool has[4];
+    reg4_corr(Q, C, 0, lane, corr, has);
+    T nb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) nb[k] = has[k] ? Q.phi[k] + corr[k] : Q.phi[k];
+    reg4_rb_from(Q, nb[0], nb[1], nb[2], nb[3], par, lane);
+    if (pair) reg4_rb(Q, 1 - par, lane);
+}
+
+// A run whose finest level lmin holds 65 to 256 points: that level in the
+// 4-point layout, up to three coarser one-point levels.
+template <typename T>
+__device__ __forceinline__ void coarse_reg_run4(const unsigned* sops, int o, int run, int lmin, int lmax,
+                                                const LvPtr<T>* __restrict__ lv, T* sm, int lane)
+{
+    RegLv4<T> Q;
+    RegLv<T> S1, S2, S3;
+    reg4_load(Q, lv[lmin], sm, lane);
+    if (lmin + 1 <= lmax) reg_load(S1, lv[lmin + 1], sm, lane);
+    if (lmin + 2 <= lmax) reg_load(S2, lv[lmin + 2], sm, lane);
+    if (lmin + 3 <= lmax) reg_load(S3, lv[lmin + 3], sm, lane);
+    unsigned wn = sops[o];
+    for (int k = 0; k < run; ++k) {
+        const CoarseOp q = coarse_unpack(__builtin_amdgcn_readfirstlane(wn));
+        if (k + 1 < run) wn = sops[o + k + 1];   // next op word, read ahead
+        const int j = q.level - lmin;
+        if (j == 0) {
+            if (q.kind == CO_RB || q.kind == CO_RB_PAIR) {
+                reg4_rb(Q, q.a, lane);
+                if (q.kind == CO_RB_PAIR) reg4_rb(Q, 1 - q.a, lane);
+            } else if (q.kind == CO_RESTRICT) {
+                reg4_restrict(Q, S1, q.b, q.a, lane);
+            } else if (q.kind == CO_PRO_RB) {
+                reg4_pro_rb(Q, S1, q.a, q.b, lane);
+            } else if (q.kind == CO_PROLONG) {
+                reg4_prolong(Q, S1, q.a, lane);
+            } else {  // CO_ZERO
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Q.phi[e] = (T)0;
+            }
+            continue;
+        }
+        if (q.kind == CO_RB || q.kind == CO_RB_PAIR) {
+            auto sweep = [&](RegLv<T>& S) {
+                reg_rb(S, q.a, lane);
+                if (q.kind == CO_RB_PAIR) reg_rb(S, 1 - q.a, lane);
+            };
+            if (j == 1) sweep(S1);
+            else if (j == 2) sweep(S2);
+            else sweep(S3);
+        } else if (q.kind == CO_RESTRICT) {
+            if (j == 1) reg_restrict(S1, S2, q.b, q.a, lane);
+            else reg_restrict(S2, S3, q.b, q.a, lane);
+        } else if (q.kind == CO_PRO_RB) {
+            if (j == 1) reg_pro_rb(S1, S2, q.a, q.b, lane);
+            else reg_pro_rb(S2, S3, q.a, q.b, lane);
+        } else if (q.kind == CO_PROLONG) {
+            if (j == 1) reg_prolong(S1, S2, q.a, lane);
+            else reg_prolong(S2, S3, q.a, lane);
+        } else {  // CO_ZERO
+            if (j == 1) S1.phi = (T)0;
+            else if (j == 2) S2.phi = (T)0;
+            else S3.phi = (T)0;
+        }
+    }
+    reg4_store(Q, lv[lmin], sm, lane);
+    if (lmin + 1 <= lmax) reg_store(S1, lv[lmin + 1], sm, lane);
+    if (lmin + 2 <= lmax) reg_store(S2, lv[lmin + 2], sm, lane);
+    if (lmin + 3 <= lmax) reg_store(S3, lv[lmin + 3], sm, lane);
+}
+
 // The run of `run` ops from word o, in wave 0: the levels it touches (at most
-// three, lmin..lmax) are loaded into registers, the ops execute there, and
-// phi and rhs go back to LDS.
+// three of one point per lane, lmin..lmax, or a 4-point level and three
+// more) are loaded into registers, the ops execute there, and phi and rhs go
+// back to LDS.
 template <typename T>
 __device__ __forceinline__ void coarse_reg_run(const unsigned* sops, int o, int run, const LvPtr<T>* __restrict__ lv,
                                                T* sm, int lane)
@@ -1908,6 +2189,11 @@ __device__ __forceinline__ void coarse_reg_run(const unsigned* sops, int o, int 
         const int hi = (q.kind == CO_RESTRICT || q.kind == CO_PROLONG || q.kind == CO_PRO_RB) ? q.level + 1 : q.level;
         lmin = q.level < lmin ? q.level : lmin;
         lmax = hi > lmax ? hi : lmax;
+    }
+    // a finest level of more than one point per lane: the 4-point layout
+    if ((long)lv[lmin].g.mx * lv[lmin].g.my > COARSE_WAVE) {
+        coarse_reg_run4<T>(sops, o, run, lmin, lmax, lv, sm, lane);
+        return;
     }
     RegLv<T> S0, S1, S2;
     reg_load(S0, lv[lmin], sm, lane);

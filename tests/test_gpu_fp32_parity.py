@@ -130,6 +130,24 @@ def test_fp32_cycle_bitexact(case, kw):
         np.testing.assert_array_equal(dp.download(0, 0, prec=F32), want, err_msg=f"call {rep}")
 
 
+@pytest.mark.parametrize("reg", [2, 1])
+@pytest.mark.parametrize("kw", CYCLES, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()) or "V")
+@pytest.mark.parametrize("case", [c for c in CASES if c[1] == 2 and not c[5]], ids=lambda c: c[0])
+def test_fp32_cycle_coarse_program_register_runs(case, kw, reg):
+    """One FP32 cycle whose coarse program starts at level 1, its one-wave
+    runs in registers: one point per lane (reg 1) or also from a level of up
+    to 256 points, a 2 x 2 block per lane (reg 2)."""
+    dp, op, phi, rhs = _setup(case, seed=11)
+    _tune(dp, COARSE_M=1 << 20, COARSE_REG=reg)
+    c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+    want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
+    for rep in range(2):
+        dp.upload(0, 0, gmg.zero_rim(phi), prec=F32)
+        dp.upload(0, 1, rhs, prec=F32)
+        dp.cycle(cfg(**kw), prec=F32)
+        np.testing.assert_array_equal(dp.download(0, 0, prec=F32), want, err_msg=f"call {rep}")
+
+
 @pytest.mark.parametrize("fuse_prolong", [0, 2])
 @pytest.mark.parametrize("kw", [dict(), dict(gamma=2, nu2=2), dict(fcycle=True)], ids=["V", "W", "F"])
 def test_fp32_cycle_fused_sweep_bitexact(kw, fuse_prolong):

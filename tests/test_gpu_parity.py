@@ -396,16 +396,20 @@ def test_cycle_coarse_program_launch_shapes(case, cyc, solo, threads, fuse):
 
 
 REG_CASES = [c for c in CASES if c[0] in ("2d_spatial_16x32", "2d_spatial_32x8", "2d_scalar")] + [
-    ("2d_spatial_128x64", 2, (128, 64), True, True, False)]
+    ("2d_spatial_128x64", 2, (128, 64), True, True, False),
+    ("2d_spatial_64", 2, (64, 64), True, True, False),
+    ("2d_spatial_128x32", 2, (128, 32), True, False, False)]
 
 
 @pytest.mark.parametrize("fuse", [1, 0])
-@pytest.mark.parametrize("reg", [1, 0])
+@pytest.mark.parametrize("reg", [2, 1, 0])
 @pytest.mark.parametrize("cyc", FUSED_PROG_CYCLES, ids=lambda c: c[0])
 @pytest.mark.parametrize("case", REG_CASES, ids=[c[0] for c in REG_CASES])
 def test_cycle_coarse_program_register_runs(case, cyc, reg, fuse):
     """2D one-wave runs with their levels in registers (neighbours by lane
-    shuffles; square and rectangular levels) and in LDS, with the restriction
+    shuffles; square and rectangular levels; reg 2: also runs from a level of
+    65 to 256 points, a 2 x 2 block per lane, down through up to three levels
+    below it) and in LDS, with the restriction
     running the coarse level's first half and the prolongation inside the first
     post-smoothing half (JSV_TUNE_FUSE_HALVES) or as separate ops: the
     oracle's bits."""
@@ -431,6 +435,7 @@ def test_coarse_tuning_limits():
     dp, _, _, _ = _setup(CASES[0], seed=1)
     assert lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_SOLO, 65) != 0
     assert lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_THREADS, 128) != 0
+    assert lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_REG, 3) != 0
 
 
 @pytest.mark.parametrize("dim,ms", [(2, (64, 64)), (3, (16, 16, 16))])
