@@ -162,7 +162,14 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         or the residual sum marches its level's planes in
  *                         the opposite order of that level's previous pair
  *                         sweep (the Infinity Cache still holds the planes
- *                         the previous sweep touched last); 0: always upward */
+ *                         the previous sweep touched last); 0: always upward
+ *   JSV_TUNE_FUSE_RESID   1 (default): the last pre-smoothing pair sweep of a
+ *                         3D level with the two-pass restriction also stores
+ *                         the residuals that restriction reads, at the red
+ *                         points of its tiles' inner rows and planes (that
+ *                         sweep marches upward); a small pass adds the tile
+ *                         edge rows and planes.  0: one residual pass over
+ *                         the level after the sweep.  Same bits either way */
 #define JSV_TUNE_SLAB_BYTES 0
 #define JSV_TUNE_TWO_PASS_MIN 1
 #define JSV_TUNE_FUSED_MIN_M 2
@@ -181,6 +188,7 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
 #define JSV_TUNE_DEVICE_STOP 15
 #define JSV_TUNE_COARSE_REG 16
 #define JSV_TUNE_ZIGZAG 17
+#define JSV_TUNE_FUSE_RESID 18
 int jsv_plan_set_tuning(jsv_plan *plan, int key, int64_t value);
 
 /* Per-level coefficients.  Replaces compute_lambda / compute_diffusion_tensor /

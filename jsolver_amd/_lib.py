@@ -32,6 +32,7 @@ JSV_TUNE_FUSE_HALVES = 14
 JSV_TUNE_DEVICE_STOP = 15
 JSV_TUNE_COARSE_REG = 16
 JSV_TUNE_ZIGZAG = 17
+JSV_TUNE_FUSE_RESID = 18
 JSV_BUF_PHI = 0
 JSV_BUF_RHS = 1
 JSV_NOALLOC_MAX_SEGMENTS = 1024

@@ -123,6 +123,7 @@ struct jsv_plan {
     int sweep_pairs = 0;            // pair kernel: column pairs per thread (0: 2 in FP32, 1 in FP64)
     int n_cu = 256;                 // compute units of the device
     int zigzag = 1;                 // a plain pair sweep marches opposite to its level's previous pair sweep
+    int fuse_resid = 1;             // the last pre-smoothing pair sweep stores the restriction's residuals
     mutable unsigned zz_dn[2] = {0u, 0u};   // FP64 / FP32: bit l = level l's last pair sweep marched downward
     int fuse_prolong = 2;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
     int fuse_halves = 1;            // levels smoothed one launch per half: the restriction runs the coarse
@@ -270,14 +271,22 @@ int launch_rbsweep3_t(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s)
 // ring row width of the pair kernel instantiated for level g (its NHMAX)
 int pair_nhm(const Geo& g) { return (g.nx + 1) / 2 <= 130 ? 130 : 258; }
 
-size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1, bool res = false)
+size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1, bool res = false,
+                      bool fr = false, int nt = 1024)
 {
     const int nhm = pair_nhm(g);
-    size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * nhm + (size_t)8 * np * 1024;
+    size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * nhm + (size_t)8 * np * nt;
     if (gc != nullptr) e += (size_t)3 * (ty / 2 + 4) * (nhm + 1);
-    if (res) e += (size_t)3 * (ty + 2) * nhm + (size_t)np * 1024;   // ring of phi_in's red values, held black values
+    if (res) e += (size_t)3 * (ty + 2) * nhm + (size_t)np * nt;     // ring of phi_in's red values, held black values
+    if (fr) e += (size_t)3 * (ty - 2) * nhm + (size_t)3 * np * nt;  // one plane's black values and y products; holds
     return w * e;
 }
+
+// A pre-smoothing pair sweep that also stored the first-half residuals of its
+// tiles' inner rows and planes (ty > 0), and the tiling the rest follows.
+struct FrGeo {
+    int ty = 0, zchunk = 0, ytiles = 0;
+};
 
 bool pair_sweep_rows_fit(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1)
 {
@@ -323,8 +332,9 @@ int pair_sweep_per_cu(const Geo& g, int ty)
 // no such variant, the sweep ran without it)
 template <typename T, int TY, int NHM, int NP>
 int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s,
-                       double* res, long* nres)
+                       double* res, long* nres, FrGeo* fr)
 {
+    if (fr != nullptr) fr->ty = 0;
     const Geo& g = H.g[l];
     constexpr int NT = 1024;
     const int nh = (g.nx + 1) / 2;
@@ -345,17 +355,42 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
     static const bool wr_pro = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, true, NP, 1, true>);
     static const bool wr_plain = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true>);
     const bool wr = ((g.mx / 2) / NP) % 64 == 0 && (pc != nullptr ? wr_pro : wr_plain);
-    auto go = [&](auto kern, const T* pcv, size_t lds_k = 0, double* rp = nullptr) {
+    auto go = [&](auto kern, const T* pcv, size_t lds_k = 0, double* rp = nullptr, T* rfo = nullptr) {
         // zig-zag: a plain sweep reverses its level's previous pair sweep
-        // (the variants with the prolongation or the residual sum march upward)
+        // (the variants with the prolongation, the residual sum or the
+        // first-half residuals march upward)
         unsigned& last = P->zz_dn[sizeof(T) == 8 ? 0 : 1];
         const unsigned bit = 1u << l;
-        const int dn = (P->zigzag && pcv == nullptr && rp == nullptr && (last & bit) == 0u) ? 1 : 0;
+        const int dn = (P->zigzag && pcv == nullptr && rp == nullptr && rfo == nullptr && (last & bit) == 0u) ? 1 : 0;
         last = dn ? (last | bit) : (last & ~bit);
         hipLaunchKernelGGL(kern, grid, dim3(NT), lds_k > 0 ? lds_k : lds, s, H.phi[l], H.phi2[l], H.rhs[l],
-                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp, dn);
+                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp, dn,
+                           rfo);
     };
     if (nres != nullptr) *nres = 0;
+    if constexpr (TY >= 4) {
+        // the first-half residuals of the tiles' inner rows and planes as
+        // well, where that variant needs no scratch memory
+        const size_t lds_f = pair_sweep_lds(g, TY, sizeof(T), nullptr, NP, false, true);
+        if (fr != nullptr && pc == nullptr && res == nullptr && H.res[l] != nullptr && P->fuse_resid &&
+            lds_f <= 160 * 1024) {
+            static const bool wr_fr =
+                no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true, false, true>);
+            static const bool rt_fr =
+                no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, false, false, true>);
+            const bool wrf = ((g.mx / 2) / NP) % 64 == 0 && wr_fr;
+            if (wrf || rt_fr) {
+                wrf ? go(jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true, false, true>, (const T*)nullptr,
+                         lds_f, nullptr, H.res[l])
+                    : go(jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, false, false, true>, (const T*)nullptr,
+                         lds_f, nullptr, H.res[l]);
+                fr->ty = TY;
+                fr->zchunk = zchunk;
+                fr->ytiles = ytiles;
+                return 0;
+            }
+        }
+    }
     if constexpr (sizeof(T) == 8 && NP == 1 && (TY == 5 || TY == 4 || TY == 2)) {
         const size_t lds_r = pair_sweep_lds(g, TY, sizeof(T), nullptr, NP, true);
         if (res != nullptr && pc == nullptr && lds_r <= 160 * 1024 && (long)grid.x <= P->npartials) {
@@ -378,23 +413,66 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
 
 template <typename T, int TY, int NP>
 int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T* pc, hipStream_t s, double* res,
-                       long* nres)
+                       long* nres, FrGeo* fr)
 {
-    if ((H.g[l].nx + 1) / 2 <= 130) return launch_rbsweep3p_t<T, TY, 130, NP>(P, H, l, zchunk, pc, s, res, nres);
-    return launch_rbsweep3p_t<T, TY, 258, NP>(P, H, l, zchunk, pc, s, res, nres);
+    if ((H.g[l].nx + 1) / 2 <= 130)
+        return launch_rbsweep3p_t<T, TY, 130, NP>(P, H, l, zchunk, pc, s, res, nres, fr);
+    return launch_rbsweep3p_t<T, TY, 258, NP>(P, H, l, zchunk, pc, s, res, nres, fr);
 }
 
 template <typename T, int NP>
 int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zchunk, const T* pc, hipStream_t s,
-                          double* res, long* nres)
+                          double* res, long* nres, FrGeo* fr)
 {
     switch (rows) {
-    case 14: return launch_rbsweep3p_t<T, 14, NP>(P, H, l, zchunk, pc, s, res, nres);
-    case 6: return launch_rbsweep3p_t<T, 6, NP>(P, H, l, zchunk, pc, s, res, nres);
-    case 5: return launch_rbsweep3p_t<T, 5, NP>(P, H, l, zchunk, pc, s, res, nres);
-    case 4: return launch_rbsweep3p_t<T, 4, NP>(P, H, l, zchunk, pc, s, res, nres);
-    default: return launch_rbsweep3p_t<T, 2, NP>(P, H, l, zchunk, pc, s, res, nres);
+    case 14: return launch_rbsweep3p_t<T, 14, NP>(P, H, l, zchunk, pc, s, res, nres, fr);
+    case 6: return launch_rbsweep3p_t<T, 6, NP>(P, H, l, zchunk, pc, s, res, nres, fr);
+    case 5: return launch_rbsweep3p_t<T, 5, NP>(P, H, l, zchunk, pc, s, res, nres, fr);
+    case 4: return launch_rbsweep3p_t<T, 4, NP>(P, H, l, zchunk, pc, s, res, nres, fr);
+    default: return launch_rbsweep3p_t<T, 2, NP>(P, H, l, zchunk, pc, s, res, nres, fr);
     }
+}
+
+// The pre-smoothing pair sweep that also stores the first-half residuals
+// (FR) with NT threads per workgroup: FP32 two-pair tiles of 10 rows take 768
+// threads, whose waves may hold 168 VGPRs (at 1024 threads and 14 rows that
+// variant spills).  Returns 1 (nothing launched) where the shape does not fit.
+template <typename T, int TY, int NHM, int NP, int NT>
+int launch_rbsweep3p_fr(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, FrGeo* fr)
+{
+    const Geo& g = H.g[l];
+    const int nhe = g.mx / 2;
+    const size_t lds = pair_sweep_lds(g, TY, sizeof(T), nullptr, NP, false, true, NT);
+    if ((g.nx + 1) / 2 > NHM || nhe % NP != 0 || (TY + 2) * (nhe / NP) > NT || lds > 160 * 1024 ||
+        (double)g.n * sizeof(T) >= 4294967296.0)
+        return 1;
+    auto no_scratch = [](const void* k) {
+        hipFuncAttributes a;
+        return hipFuncGetAttributes(&a, k) == hipSuccess && a.localSizeBytes == 0;
+    };
+    const auto kwr = jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true, false, true>;
+    const auto krt = jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, false, false, true>;
+    static const bool wr_ok = no_scratch((const void*)kwr), rt_ok = no_scratch((const void*)krt);
+    const bool wr = (nhe / NP) % 64 == 0 && wr_ok;
+    if (!wr && !rt_ok) return 1;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wr ? kwr : krt, NT, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int ytiles = (g.my - 1 + TY - 1) / TY;
+    int zc = P->n_cu * per_cu / ytiles;
+    if (zc < 1) zc = 1;
+    if (zc > g.mz - 1) zc = g.mz - 1;
+    const int zchunk = P->zchunk > 0 ? P->zchunk : (g.mz - 1 + zc - 1) / zc;
+    const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
+    // an upward march: the level's next plain sweep reverses it
+    P->zz_dn[sizeof(T) == 8 ? 0 : 1] &= ~(1u << l);
+    hipLaunchKernelGGL(wr ? kwr : krt, dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds, s, H.phi[l], H.phi2[l],
+                       H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, (const T*)nullptr, g,
+                       P->act_ptr(), (double*)nullptr, 0, H.res[l]);
+    fr->ty = TY;
+    fr->zchunk = zchunk;
+    fr->ytiles = ytiles;
+    return 0;
 }
 
 // Pairs per thread, rows and z-chunk of the pair sweep.  Two pairs per thread
@@ -405,7 +483,7 @@ int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zc
 // then gives each slot one tile.
 template <typename T>
 int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s, double* res = nullptr,
-                     long* nres = nullptr)
+                     long* nres = nullptr, FrGeo* fr = nullptr)
 {
     const Geo& g = H.g[l];
     const Geo* gc = pc != nullptr ? &H.g[l + 1] : nullptr;
@@ -414,6 +492,14 @@ int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStrea
     // tuning forces
     int np = P->sweep_pairs > 0 ? P->sweep_pairs : ((sizeof(T) == 4 && g.mx >= 256) ? 2 : 1);
     if (np == 2 && !pair_sweep_rows_fit(g, 2, sizeof(T), gc, 2)) np = 1;
+    if constexpr (sizeof(T) == 4) {
+        if (fr != nullptr && np == 2 && pc == nullptr && res == nullptr && H.res[l] != nullptr && P->fuse_resid &&
+            (P->sweep_rows == 0 || P->sweep_rows == 10)) {
+            const int rc = (g.nx + 1) / 2 <= 130 ? launch_rbsweep3p_fr<T, 10, 130, 2, 768>(P, H, l, s, fr)
+                                                 : launch_rbsweep3p_fr<T, 10, 258, 2, 768>(P, H, l, s, fr);
+            if (rc == 0) return 0;
+        }
+    }
     int rows = P->sweep_rows;
     if (rows == 8 || rows == 16) rows = 6;
     const bool tuned = rows > 0 && pair_sweep_rows_fit(g, rows, sizeof(T), gc, np);
@@ -449,19 +535,20 @@ int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStrea
         }
         zchunk = (g.mz - 1 + zchunks - 1) / zchunks;
     }
-    if (np == 2) return launch_rbsweep3p_rows<T, 2>(P, H, l, rows, zchunk, pc, s, res, nres);
-    return launch_rbsweep3p_rows<T, 1>(P, H, l, rows, zchunk, pc, s, res, nres);
+    if (np == 2) return launch_rbsweep3p_rows<T, 2>(P, H, l, rows, zchunk, pc, s, res, nres, fr);
+    return launch_rbsweep3p_rows<T, 1>(P, H, l, rows, zchunk, pc, s, res, nres, fr);
 }
 
 // tile rows x threads per workgroup of the fused sweep (tunable; default by precision)
 template <typename T>
 int launch_rbsweep3(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr,
-                    double* res = nullptr, long* nres = nullptr)
+                    double* res = nullptr, long* nres = nullptr, FrGeo* fr = nullptr)
 {
     if (nres != nullptr) *nres = 0;
+    if (fr != nullptr) fr->ty = 0;
     if (pc != nullptr) return launch_rbsweep3p<T>(P, H, l, pc, s);
     if (P->sweep_variant == 2 && pair_sweep_fits(H.g[l], sizeof(T)))
-        return launch_rbsweep3p<T>(P, H, l, nullptr, s, res, nres);
+        return launch_rbsweep3p<T>(P, H, l, nullptr, s, res, nres, fr);
     int rows = P->sweep_rows > 0 ? P->sweep_rows : (sizeof(T) == 4 ? 16 : 4);
     const int nt = P->sweep_threads > 0 ? P->sweep_threads : 1024;
     if (nt == 256) {
@@ -507,12 +594,15 @@ bool halves_path(const jsv_plan* P, const Hier<T>& H, int l)
 // this level (fuse_halves; halves path only), so only the second half runs.
 // res / nres: see launch_rbsweep3p_t (fused out-of-place sweeps only; *nres
 // is 0 where the residual was not summed)
+// fr: the sweep may also store the first-half residuals of the swept phi for
+// the restriction that follows (*fr says which ones; ty = 0: none)
 template <typename T>
 int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nullptr, bool red_done = false,
-           double* res = nullptr, long* nres = nullptr)
+           double* res = nullptr, long* nres = nullptr, FrGeo* fr = nullptr)
 {
     const Geo& g = H.g[l];
     if (nres != nullptr) *nres = 0;
+    if (fr != nullptr) fr->ty = 0;
     if (P->dim == 2) {
         if (P->offdiag) {
             // the reference's sub-colour order (0,0),(1,1),(0,1),(1,0)
@@ -539,7 +629,7 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
         }
     } else if (P->fused_min_m > 0 && g.mz >= P->fused_min_m && H.phi2[l] != nullptr) {
         // fused out-of-place sweep, then swap the level's phi buffers
-        RETCHK(launch_rbsweep3<T>(P, H, l, s, pc, res, nres));
+        RETCHK(launch_rbsweep3<T>(P, H, l, s, pc, res, nres, fr));
         std::swap(H.phi[l], H.phi2[l]);
     } else {
         // Slab schedule: red on z-slab k, then black on slab k-1, so the black
@@ -585,9 +675,12 @@ int smooth(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, const T* pc = nu
 }
 
 // zphi: the coarse phi to zero in the same kernels (nullptr: left alone);
-// red_first: also run the coarse level's first RBGS half (zphi must be set)
+// red_first: also run the coarse level's first RBGS half (zphi must be set);
+// fr: the sweep before stored the first-half residuals of its tiles' inner
+// rows and planes (fr->ty > 0), so only the edge rows and planes remain
 template <typename T>
-int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nullptr, bool red_first = false)
+int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nullptr, bool red_first = false,
+              const FrGeo* fr = nullptr)
 {
     const Geo& gf = H.g[l];
     const Geo& gc = H.g[l + 1];
@@ -606,7 +699,13 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
             hipLaunchKernelGGL(jsv::k_restrict_gather2<T>, gr, dim3(jsv::BX, jsv::BY), 0, s,
                                H.rhs[l + 1], H.res[l], gf, gc, P->act_ptr(), zphi, rz);
         } else {
-            if (gf.n < (1L << 24)) {
+            if (fr != nullptr && fr->ty > 0) {
+                const int zchunks = (gf.mz - 1 + fr->zchunk - 1) / fr->zchunk;
+                const long nb = (long)(gf.mz - 1) * 2 * fr->ytiles + (long)2 * zchunks * (gf.my - 1);
+                hipLaunchKernelGGL(jsv::k_resid_edges3<T>, dim3((unsigned)nb), dim3(jsv::EDGE_THREADS), 0, s,
+                                   H.res[l], H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, fr->ty,
+                                   fr->zchunk, fr->ytiles, P->act_ptr());
+            } else if (gf.n < (1L << 24)) {
                 hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
                                    H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 1, P->act_ptr());
             } else {
@@ -1114,10 +1213,14 @@ int mg_cycle_body(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, boo
         }
         return 0;
     }
+    // the last pre-smoothing sweep may store the restriction's residuals too
+    FrGeo frg;
     if (!skip_pre)
         for (int i = 0; i < c.nu1; ++i) {
             if (i == 0 && test_first) RETCHK(first_sweep());
-            else RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0));
+            else
+                RETCHK(smooth(P, H, l, s, (const T*)nullptr, red_done && i == 0, nullptr, nullptr,
+                              i == c.nu1 - 1 ? &frg : nullptr));
         }
     const int start = coarse_start(P, H);
     const jsv_plan::CoarseProg* pr = (start == l + 1) ? find_prog<T>(P, c, fcyc, start) : nullptr;
@@ -1128,7 +1231,7 @@ int mg_cycle_body(const jsv_plan* P, Hier<T>& H, const jsv_config& c, int l, boo
     const int first_sweeps = (l + 1 == P->levels - 2) ? c.nu1 + c.nu2 : c.nu1;
     const bool red_first = pr == nullptr && P->fuse_halves && first_sweeps >= 1 && halves_path(P, H, l + 1);
     // phi of level l + 1 := 0 happens inside the restriction (or the program)
-    RETCHK(restrict_(P, H, l, s, pr != nullptr ? (T*)nullptr : H.phi[l + 1], red_first));
+    RETCHK(restrict_(P, H, l, s, pr != nullptr ? (T*)nullptr : H.phi[l + 1], red_first, &frg));
     if (pr != nullptr) {
         // zero + the gamma (F: 2 gamma) sub-cycles of level l + 1 in one launch
         const int pi = sizeof(T) == 4 ? 1 : 0;
@@ -1586,6 +1689,10 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
     }
     if (key == JSV_TUNE_ZIGZAG) {
         P->zigzag = value != 0 ? 1 : 0;
+        return 0;
+    }
+    if (key == JSV_TUNE_FUSE_RESID) {
+        P->fuse_resid = value != 0 ? 1 : 0;
         return 0;
     }
     if (key == JSV_TUNE_COARSE_REG) {

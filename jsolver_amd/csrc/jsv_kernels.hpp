@@ -528,6 +528,14 @@ k_rbsweep3(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict_
 // sweep reads).  Requires (TY + 2) * (mx / 2) <= NT, (nx + 1) / 2 <= NHMAX
 // and the level below 4 GiB; rim values of phi_in are zero.
 // ---------------------------------------------------------------------------
+// First-half residuals of a 3D level are stored compactly: point (z, y, x)
+// with x + y + z odd, z, y, x in 2..m, at ((z-2)(my-1) + y-2) * mx/2 + (x-2)/2
+// (x = 2 + parity + 2i, so (x-2)/2 = i): half the bytes of a full array.
+__device__ __forceinline__ long res3_at(const Geo& g, int z, int y, int x)
+{
+    return ((long)(z - 2) * (g.my - 1) + (y - 2)) * (g.mx >> 1) + ((x - 2) >> 1);
+}
+
 // sums / maxima over the 64 lanes of a wave (every lane gets the result)
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -564,15 +572,16 @@ struct PairGrp {
 };
 
 template <typename T, int TY, int NT, int NHMAX, bool PRO = false, int NP = 1, int WPE = 1, bool WR = false,
-          bool RES = false>
+          bool RES = false, bool FR = false>
 __global__ void __launch_bounds__(NT, WPE)
 k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
             const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
             const T* __restrict__ cc, Geo g, int zchunk, int ytiles, const T* __restrict__ pc, Geo gc,
-            const int* __restrict__ act, double* __restrict__ res_part, int dn_in)
+            const int* __restrict__ act, double* __restrict__ res_part, int dn_in, T* __restrict__ rfo)
 {
     if (act != nullptr && act[0] == 0) return;
     static_assert(!(RES && PRO), "the residual is of phi_in as given");
+    static_assert(!(FR && (PRO || RES)), "the first-half residuals come from the plain sweep");
     constexpr int PR = TY + 4;                  // rows of the phi_in ring: y0-2 .. y0+TY+1
     constexpr int RR = TY + 2;                  // rows of the red ring / pair rows: y0-1 .. y0+TY
     constexpr int PMAX = (PR * NHMAX + NT - 1) / NT;   // ring elements per thread (nh <= NHMAX)
@@ -590,6 +599,15 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     T* Cr = Hb + 8 * NP * NT;                   // PRO: [3][CRR][gc.nx] coarse planes of the correction
     T* Ro = Cr + (PRO ? 3 * CRR * (NHMAX + 1) : 0);   // RES: [3][RR][nh] red values of phi_in
     T* Pb = Ro + 3 * RR * NHMAX;                // RES: [NP][NT] this thread's held black values of phi_in
+    // FR: for plane zs - 1 and each red row y0 + 1 .. y0 + TY - 2 (row i =
+    // y - y0 - 1), the black values of that row (the residual's x
+    // neighbours) and the products of the black points of the rows below and
+    // above with their faces toward it (its y terms); per thread, the red rhs
+    // of the last two planes and one partial residual per pair
+    T* Bk = Ro;                                 // FR: [TY - 2][nh] phi of row y
+    T* Bu = Bk + (TY - 2) * NHMAX;              // FR: [TY - 2][nh] cy(y - 1) * phi(y - 1)
+    T* Bd = Bu + (TY - 2) * NHMAX;              // FR: [TY - 2][nh] cy(y) * phi(y + 1)
+    T* Hf = Bd + (TY - 2) * NHMAX;              // FR: [3][NP][NT]
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int y0 = 2 + (tile % ytiles) * TY;
@@ -601,6 +619,8 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     for (int i = tid; i < 3 * RR * nh; i += NT) R[i] = (T)0;
     if constexpr (RES)
         for (int i = tid; i < 3 * RR * nh; i += NT) Ro[i] = (T)0;
+    if constexpr (FR)
+        for (int i = tid; i < 3 * (TY - 2) * nh; i += NT) Bk[i] = (T)0;
 
     const bool has = tid < RR * gpr;
     const int r = has ? tid / gpr : 0;
@@ -626,7 +646,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // so the bits are those of the upward march.  The variants with the
     // prolongation or the residual sum (whose term orders are not symmetric in
     // z) always march upward.
-    const int dn = (PRO || RES) ? 0 : dn_in;
+    const int dn = (PRO || RES || FR) ? 0 : dn_in;
     auto pl = [&](int u) -> int { return dn ? g.nz - 1 - u : u; };
     auto plz = [&](int u) -> int { return dn ? max(g.nz - 2 - u, 0) : u; };
     auto at = [](const T* base, unsigned boff) -> T {
@@ -812,6 +832,28 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     T red_prev[NP];
 #pragma unroll
     for (int pp = 0; pp < NP; ++pp) red_prev[pp] = (T)0;
+    // FR: the residual rhs - L phi at the red points (x + y + z odd) of the
+    // swept level, in resid3's term order, written to rfo at res3_at.  The
+    // residual of plane z needs the black values of planes z - 1, z and
+    // z + 1, so it is formed over two steps: cc * phi and the z - 1 product
+    // when the red value is computed (step z), the four in-plane terms and
+    // the z - 1 term after the black half of plane z (step z + 1), the z + 1
+    // term and rhs after the black half of plane z + 1 (step z + 2).  Only
+    // the tile's rows y0 + 1 .. y0 + TY - 2 and planes z0 + 1 .. z1 - 2 are
+    // done here: the other rows' and planes' neighbours belong to other
+    // tiles (k_resid_edges3 covers those points).  Products are formed
+    // early, subtractions in resid3's order, so the bits are resid3's.
+    T t0h[NP], pmh[NP];                         // cc * phi and the z - 1 product of plane zs - 1's red points
+    T cxe = (T)0;                               // cx of plane zs - 1's red point at the thread's edge
+#pragma unroll
+    for (int pp = 0; pp < NP; ++pp) {
+        t0h[pp] = (T)0;
+        pmh[pp] = (T)0;
+    }
+    // 0, 1: red rhs by plane parity; 2: plane zs - 2's partial residual
+    auto HF = [&](int j, int pp) -> T& { return Hf[(j * NP + pp) * NT + tid]; };
+    const bool fr_row = FR && inner && r >= 2 && r <= TY - 1;
+    auto fr_plane = [&](int u) { return u >= z0 + 1 && u <= z1 - 2; };
 
     // ec: which element of a pair is red on this step's plane, as a compile-time
     // constant (see the loop below)
@@ -890,6 +932,9 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         __syncthreads();
         // ---- black on plane zb = zs - 1 at x = xr; write both colours of the pairs
         const int zb = zs - 1;
+        T pmn[NP];                              // FR
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp) pmn[pp] = (T)0;
         if (inner && zb >= z0 && zb < z1) {
             T* orow = pout + ((long)pl(zb) * sz + (long)y * sy);
             // both colours of the thread's pairs leave as one vector store; a
@@ -926,11 +971,64 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                         rsum += (double)rr * (double)rr;
                     }
                 }
+                if constexpr (FR) {
+                    // this black row is y - y0 = r - 1: row r - 2 of Bk, the row
+                    // above's Bu, the row below's Bd
+                    const int col = k + pp;
+                    if (r >= 2 && r <= TY - 1) Bk[(r - 2) * nh + col] = blk;
+                    if (r <= TY - 2) Bu[(r - 1) * nh + col] = H(pp, 2) * blk;
+                    if (r >= 3) Bd[(r - 3) * nh + col] = H(pp, 3) * blk;
+                    // the z - 1 product of plane zs's red point (same x)
+                    pmn[pp] = JSV_EL(czp, pp, e) * blk;
+                    // plane zs - 2's red point (same x): its z + 1 term (cz of that
+                    // point is held as this black point's z - 1 face) and rhs
+                    if (fr_row && fr_plane(zs - 2) && xr <= g.mx) {
+                        T rr = HF(2, pp) - H(pp, 1) * blk;
+                        rr = rr + HF(zs & 1, pp);
+                        rfo[res3_at(g, zs - 2, y, xr)] = rr;
+                    }
+                }
                 const T rp = ((xr ^ 1) <= g.mx) ? red_prev[pp] : (T)0;
                 ov.v[2 * pp] = e ? rp : blk;
                 ov.v[2 * pp + 1] = e ? blk : rp;
             }
             *reinterpret_cast<PairVec<T, 2 * NP>*>(orow + 2 * k) = ov;
+        } else if (FR && has && y == g.my + 1 && r >= 3 && r <= TY && zb >= z0 && zb < z1) {
+            // the boundary row above the last unknown row: the y + 1 terms of
+            // that row's red points multiply cy by this row's zero phi
+#pragma unroll
+            for (int pp = 0; pp < NP; ++pp) Bd[(r - 3) * nh + k + pp] = H(pp, 3) * (T)0;
+        }
+        if constexpr (FR) {
+            __syncthreads();                    // every row's black values of plane zs - 1 are in Bk, Bu, Bd
+            // plane zs - 1's red points at x = 2(k + pp) + 1 - e: cc * phi, the
+            // in-plane terms, the z - 1 term.  Their cx faces are held by this
+            // thread's black halves of the same plane (cxe at its edge).
+            if (fr_row && fr_plane(zs - 1)) {
+#pragma unroll
+                for (int pp = 0; pp < NP; ++pp) {
+                    const int x = 2 * (k + pp) + 1 - e;
+                    if (x <= g.mx) {
+                        T fxm, fxp;
+                        if (e == 1) {
+                            fxp = H(pp, 5);
+                            fxm = pp > 0 ? H(pp - 1, 4) : cxe;
+                        } else {
+                            fxm = H(pp, 4);
+                            fxp = pp + 1 < NP ? H(pp + 1, 5) : cxe;
+                        }
+                        const int row = (r - 2) * nh;
+                        const T bxm = Bk[row + ((x - 1) >> 1)], bxp = Bk[row + ((x + 1) >> 1)];
+                        const T pym = Bu[row + (x >> 1)], pyp = Bd[row + (x >> 1)];
+                        T rr = t0h[pp] - fxm * bxm;
+                        rr = rr - fxp * bxp;
+                        rr = rr - pym;
+                        rr = rr - pyp;
+                        rr = rr - pmh[pp];
+                        HF(2, pp) = rr;
+                    }
+                }
+            }
         }
         // ---- keep plane zs's black halves (the points that are not red)
 #pragma unroll
@@ -945,8 +1043,14 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
             H(pp, 6) = JSV_EL(cur.rh, pp, b1);
             H(pp, 7) = JSV_EL(cur.dg, pp, b1);
             if constexpr (RES) Pbv(pp) = JSV_EL(ph, pp, b1);
+            if constexpr (FR) {
+                HF(zs & 1, pp) = JSV_EL(cur.rh, pp, e);
+                t0h[pp] = JSV_EL(cur.dg, pp, e) * red[pp];
+                pmh[pp] = pmn[pp];
+            }
             red_prev[pp] = red[pp];
         }
+        if constexpr (FR) cxe = e ? cur.cx.v[2 * NP - 1] : cur.cxm;
 #undef JSV_EL
 #undef JSV_XM
         czp = cur.cz;
@@ -1179,13 +1283,6 @@ k_restrict3(T* __restrict__ rc, const T* __restrict__ phi, const T* __restrict__
 // reads) once; pass 2 combines them per coarse unknown in the reference's
 // order.  Same bits as the one-pass kernels above, without evaluating each
 // edge residual four times.
-// First-half residuals of a 3D level are stored compactly: point (z, y, x)
-// with x + y + z odd, z, y, x in 2..m, at ((z-2)(my-1) + y-2) * mx/2 + (x-2)/2
-// (x = 2 + parity + 2i, so (x-2)/2 = i): half the bytes of a full array.
-__device__ __forceinline__ long res3_at(const Geo& g, int z, int y, int x)
-{
-    return ((long)(z - 2) * (g.my - 1) + (y - 2)) * (g.mx >> 1) + ((x - 2) >> 1);
-}
 
 // RB rows per block: tall blocks on large levels, so the rows above and below
 // a block (read for the y-neighbours) are few against the rows it owns.
@@ -1206,6 +1303,45 @@ k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
     if (x > g.mx) return;
     const long p = (long)z * g.sz + (long)y * g.sy + x;
     r[res3_at(g, z, y, x)] = resid3<T, T>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);
+}
+
+// The first-half residuals a pre-smoothing pair sweep with FR leaves out: the
+// first and last row of each of its ty-row tiles (every plane), and the first
+// and last plane of each z-chunk (the other rows).  One workgroup per (plane,
+// row): blocks [0, (mz - 1) * 2 * ytiles) take the tile edge rows, the rest
+// the chunk edge planes; resid3 at each red point, as k_resid_first3.
+constexpr int EDGE_THREADS = 128;
+template <typename T>
+__global__ void __launch_bounds__(EDGE_THREADS)
+k_resid_edges3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
+               const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+               const T* __restrict__ cc, Geo g, int ty, int zchunk, int ytiles, const int* __restrict__ act)
+{
+    if (act != nullptr && act[0] == 0) return;
+    const int nA = (g.mz - 1) * 2 * ytiles;
+    int b = blockIdx.x, z, y;
+    if (b < nA) {
+        const int zi = b / (2 * ytiles), j = b - zi * 2 * ytiles;
+        z = 2 + zi;
+        y = 2 + (j >> 1) * ty + ((j & 1) ? ty - 1 : 0);
+        if (y > g.my) return;
+    } else {
+        b -= nA;
+        const int j = b / (g.my - 1);
+        y = 2 + (b - j * (g.my - 1));
+        const int z0 = 2 + (j >> 1) * zchunk;
+        const int z1 = min(z0 + zchunk, g.mz + 1);
+        z = (j & 1) ? z1 - 1 : z0;
+        if ((j & 1) && z == z0) return;           // a one-plane chunk: done once
+        const int ry = (y - 2) % ty;
+        if (ry == 0 || ry == ty - 1) return;      // a tile edge row: the first blocks have it
+    }
+    const int x0 = 2 + ((1 + y + z) & 1);         // x + y + z odd
+    for (int i = threadIdx.x; x0 + 2 * i <= g.mx; i += EDGE_THREADS) {
+        const int x = x0 + 2 * i;
+        const long p = (long)z * g.sz + (long)y * g.sy + x;
+        r[res3_at(g, z, y, x)] = resid3<T, T>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);
+    }
 }
 
 template <typename T>

@@ -148,6 +148,28 @@ def test_fp32_cycle_coarse_program_register_runs(case, kw, reg):
         np.testing.assert_array_equal(dp.download(0, 0, prec=F32), want, err_msg=f"call {rep}")
 
 
+@pytest.mark.parametrize("fr", [1, 0])
+@pytest.mark.parametrize("rows,zchunk", [(0, 0), (5, 3), (14, 0), (6, 1)], ids=lambda v: str(v))
+@pytest.mark.parametrize("ms,pairs", [((256, 64, 64), 2), ((64, 32, 16), 2), ((32, 16, 64), 1)],
+                         ids=lambda v: str(v))
+def test_fp32_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr):
+    """FP32 cycles whose last pre-smoothing pair sweep also stores the
+    restriction's first-half residuals (k_resid_edges3 adds the tile edges),
+    against the oracle run in float32; 256 wide with two pairs per thread is
+    the 256^3 solve's sweep shape (rows of whole waves)."""
+    case = ("3d_fr32", 3, ms, True, True, False)
+    dp, op, phi, rhs = _setup(case, seed=41)
+    _tune(dp, FUSED_MIN_M=2, TWO_PASS_MIN=1, SWEEP_PAIRS=pairs, SWEEP_ROWS=rows, ZCHUNK=zchunk, FUSE_RESID=fr)
+    for kw in (dict(), dict(gamma=2, nu2=2)):
+        c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
+        want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
+        for rep in range(2):
+            dp.upload(0, 0, gmg.zero_rim(phi), prec=F32)
+            dp.upload(0, 1, rhs, prec=F32)
+            dp.cycle(cfg(**kw), prec=F32)
+            np.testing.assert_array_equal(dp.download(0, 0, prec=F32), want, err_msg=f"{kw} call {rep}")
+
+
 @pytest.mark.parametrize("fuse_prolong", [0, 2])
 @pytest.mark.parametrize("kw", [dict(), dict(gamma=2, nu2=2), dict(fcycle=True)], ids=["V", "W", "F"])
 def test_fp32_cycle_fused_sweep_bitexact(kw, fuse_prolong):
