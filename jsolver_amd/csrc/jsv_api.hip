@@ -278,14 +278,14 @@ size_t pair_sweep_lds(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, i
     size_t e = (size_t)(3 * (ty + 4) + 3 * (ty + 2)) * nhm + (size_t)8 * np * nt;
     if (gc != nullptr) e += (size_t)3 * (ty / 2 + 4) * (nhm + 1);
     if (res) e += (size_t)3 * (ty + 2) * nhm + (size_t)np * nt;     // ring of phi_in's red values, held black values
-    if (fr) e += (size_t)3 * (ty - 2) * nhm + (size_t)3 * np * nt;  // one plane's black values and y products; holds
+    if (fr) e += (size_t)3 * (ty - 2) * nhm + (size_t)4 * np * nt;  // one plane's black values and y products; holds
     return w * e;
 }
 
 // A pre-smoothing pair sweep that also stored the first-half residuals of its
 // tiles' inner rows and planes (ty > 0), and the tiling the rest follows.
 struct FrGeo {
-    int ty = 0, zchunk = 0, ytiles = 0;
+    int ty = 0, zchunk = 0, ytiles = 0, dn = 0;   // dn: the sweep marched downward
 };
 
 bool pair_sweep_rows_fit(const Geo& g, int ty, size_t w, const Geo* gc = nullptr, int np = 1)
@@ -356,13 +356,14 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
     static const bool wr_plain = no_scratch((const void*)jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true>);
     const bool wr = ((g.mx / 2) / NP) % 64 == 0 && (pc != nullptr ? wr_pro : wr_plain);
     auto go = [&](auto kern, const T* pcv, size_t lds_k = 0, double* rp = nullptr, T* rfo = nullptr) {
-        // zig-zag: a plain sweep reverses its level's previous pair sweep
-        // (the variants with the prolongation, the residual sum or the
-        // first-half residuals march upward)
+        // zig-zag: a plain sweep (with or without the first-half residuals)
+        // reverses its level's previous pair sweep (the variants with the
+        // prolongation or the residual sum march upward)
         unsigned& last = P->zz_dn[sizeof(T) == 8 ? 0 : 1];
         const unsigned bit = 1u << l;
-        const int dn = (P->zigzag && pcv == nullptr && rp == nullptr && rfo == nullptr && (last & bit) == 0u) ? 1 : 0;
+        const int dn = (P->zigzag && pcv == nullptr && rp == nullptr && (last & bit) == 0u) ? 1 : 0;
         last = dn ? (last | bit) : (last & ~bit);
+        if (fr != nullptr) fr->dn = dn;
         hipLaunchKernelGGL(kern, grid, dim3(NT), lds_k > 0 ? lds_k : lds, s, H.phi[l], H.phi2[l], H.rhs[l],
                            H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp, dn,
                            rfo);
@@ -464,14 +465,18 @@ int launch_rbsweep3p_fr(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, FrG
     if (zc > g.mz - 1) zc = g.mz - 1;
     const int zchunk = P->zchunk > 0 ? P->zchunk : (g.mz - 1 + zc - 1) / zc;
     const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
-    // an upward march: the level's next plain sweep reverses it
-    P->zz_dn[sizeof(T) == 8 ? 0 : 1] &= ~(1u << l);
+    // zig-zag: the reverse of the level's previous pair sweep
+    unsigned& last = P->zz_dn[sizeof(T) == 8 ? 0 : 1];
+    const unsigned bit = 1u << l;
+    const int dn = (P->zigzag && (last & bit) == 0u) ? 1 : 0;
+    last = dn ? (last | bit) : (last & ~bit);
     hipLaunchKernelGGL(wr ? kwr : krt, dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds, s, H.phi[l], H.phi2[l],
                        H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, (const T*)nullptr, g,
-                       P->act_ptr(), (double*)nullptr, 0, H.res[l]);
+                       P->act_ptr(), (double*)nullptr, dn, H.res[l]);
     fr->ty = TY;
     fr->zchunk = zchunk;
     fr->ytiles = ytiles;
+    fr->dn = dn;
     return 0;
 }
 
@@ -701,10 +706,10 @@ int restrict_(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, T* zphi = nul
         } else {
             if (fr != nullptr && fr->ty > 0) {
                 const int zchunks = (gf.mz - 1 + fr->zchunk - 1) / fr->zchunk;
-                const long nb = (long)(gf.mz - 1) * 2 * fr->ytiles + (long)2 * zchunks * (gf.my - 1);
+                const long nb = (long)(fr->ytiles + 1) * (gf.mz - 1) + (long)(zchunks + 1) * (gf.my - 1);
                 hipLaunchKernelGGL(jsv::k_resid_edges3<T>, dim3((unsigned)nb), dim3(jsv::EDGE_THREADS), 0, s,
                                    H.res[l], H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, fr->ty,
-                                   fr->zchunk, fr->ytiles, P->act_ptr());
+                                   fr->zchunk, fr->ytiles, fr->dn, P->act_ptr());
             } else if (gf.n < (1L << 24)) {
                 hipLaunchKernelGGL(jsv::k_resid_first3<T>, gf_half, dim3(jsv::BX, jsv::BY), 0, s, H.res[l],
                                    H.phi[l], H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], gf, 1, P->act_ptr());

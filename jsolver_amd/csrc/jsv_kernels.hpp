@@ -607,7 +607,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     T* Bk = Ro;                                 // FR: [TY - 2][nh] phi of row y
     T* Bu = Bk + (TY - 2) * NHMAX;              // FR: [TY - 2][nh] cy(y - 1) * phi(y - 1)
     T* Bd = Bu + (TY - 2) * NHMAX;              // FR: [TY - 2][nh] cy(y) * phi(y + 1)
-    T* Hf = Bd + (TY - 2) * NHMAX;              // FR: [3][NP][NT]
+    T* Hf = Bd + (TY - 2) * NHMAX;              // FR: [4][NP][NT]
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int y0 = 2 + (tile % ytiles) * TY;
@@ -645,8 +645,9 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // terms of a stencil sum swap places, and that first sum is commutative,
     // so the bits are those of the upward march.  The variants with the
     // prolongation or the residual sum (whose term orders are not symmetric in
-    // z) always march upward.
-    const int dn = (PRO || RES || FR) ? 0 : dn_in;
+    // z) always march upward; the first-half residuals (FR) take their two z
+    // terms in stored-plane order.
+    const int dn = (PRO || RES) ? 0 : dn_in;
     auto pl = [&](int u) -> int { return dn ? g.nz - 1 - u : u; };
     auto plz = [&](int u) -> int { return dn ? max(g.nz - 2 - u, 0) : u; };
     auto at = [](const T* base, unsigned boff) -> T {
@@ -842,7 +843,10 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // the tile's rows y0 + 1 .. y0 + TY - 2 and planes z0 + 1 .. z1 - 2 are
     // done here: the other rows' and planes' neighbours belong to other
     // tiles (k_resid_edges3 covers those points).  Products are formed
-    // early, subtractions in resid3's order, so the bits are resid3's.
+    // early, subtractions in resid3's order, so the bits are resid3's.  On a
+    // downward march the stored plane below is the march's next one: the z
+    // term taken at step z + 2 comes first, and the product with the march's
+    // previous plane waits one more step.
     T t0h[NP], pmh[NP];                         // cc * phi and the z - 1 product of plane zs - 1's red points
     T cxe = (T)0;                               // cx of plane zs - 1's red point at the thread's edge
 #pragma unroll
@@ -850,7 +854,8 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         t0h[pp] = (T)0;
         pmh[pp] = (T)0;
     }
-    // 0, 1: red rhs by plane parity; 2: plane zs - 2's partial residual
+    // 0, 1: red rhs by plane parity; 2: plane zs - 2's partial residual;
+    // 3 (downward march): plane zs - 2's product with plane zs - 3
     auto HF = [&](int j, int pp) -> T& { return Hf[(j * NP + pp) * NT + tid]; };
     const bool fr_row = FR && inner && r >= 2 && r <= TY - 1;
     auto fr_plane = [&](int u) { return u >= z0 + 1 && u <= z1 - 2; };
@@ -984,8 +989,9 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                     // point is held as this black point's z - 1 face) and rhs
                     if (fr_row && fr_plane(zs - 2) && xr <= g.mx) {
                         T rr = HF(2, pp) - H(pp, 1) * blk;
+                        if (dn) rr = rr - HF(3, pp);
                         rr = rr + HF(zs & 1, pp);
-                        rfo[res3_at(g, zs - 2, y, xr)] = rr;
+                        rfo[res3_at(g, pl(zs - 2), y, xr)] = rr;
                     }
                 }
                 const T rp = ((xr ^ 1) <= g.mx) ? red_prev[pp] : (T)0;
@@ -1024,7 +1030,8 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                         rr = rr - fxp * bxp;
                         rr = rr - pym;
                         rr = rr - pyp;
-                        rr = rr - pmh[pp];
+                        if (dn) HF(3, pp) = pmh[pp];
+                        else rr = rr - pmh[pp];
                         HF(2, pp) = rr;
                     }
                 }
@@ -1306,38 +1313,57 @@ k_resid_first3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict
 }
 
 // The first-half residuals a pre-smoothing pair sweep with FR leaves out: the
-// first and last row of each of its ty-row tiles (every plane), and the first
-// and last plane of each z-chunk (the other rows).  One workgroup per (plane,
-// row): blocks [0, (mz - 1) * 2 * ytiles) take the tile edge rows, the rest
-// the chunk edge planes; resid3 at each red point, as k_resid_first3.
-constexpr int EDGE_THREADS = 128;
+// first and last row (if it exists) of each of its ty-row tiles, on every
+// plane, and the first and last plane of each z-chunk (the other rows).  The
+// edge rows come in adjacent pairs at the tile boundaries (last row of tile
+// j - 1, first row of tile j), the edge planes likewise: a workgroup takes one
+// such pair on one plane (one row), its two halves a row (plane) each, and
+// consecutive workgroups step the plane (row), so the rows and planes they
+// share are read once (XCD k runs the k-th eighth of that order, so they
+// share its L2).  Blocks [0, (ytiles + 1)(mz - 1)) take the rows;
+// resid3 at each red point, as k_resid_first3.
+// dn: the sweep marched downward (march plane u is stored plane mz + 2 - u).
+constexpr int EDGE_THREADS = 256;
 template <typename T>
 __global__ void __launch_bounds__(EDGE_THREADS)
 k_resid_edges3(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
                const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-               const T* __restrict__ cc, Geo g, int ty, int zchunk, int ytiles, const int* __restrict__ act)
+               const T* __restrict__ cc, Geo g, int ty, int zchunk, int ytiles, int dn,
+               const int* __restrict__ act)
 {
     if (act != nullptr && act[0] == 0) return;
-    const int nA = (g.mz - 1) * 2 * ytiles;
-    int b = blockIdx.x, z, y;
+    const int h = threadIdx.x >> 7, lane = threadIdx.x & 127;
+    const int nA = (ytiles + 1) * (g.mz - 1);
+    int b = xcd_tile((int)blockIdx.x, (int)gridDim.x), z, y;
     if (b < nA) {
-        const int zi = b / (2 * ytiles), j = b - zi * 2 * ytiles;
-        z = 2 + zi;
-        y = 2 + (j >> 1) * ty + ((j & 1) ? ty - 1 : 0);
-        if (y > g.my) return;
+        const int j = b / (g.mz - 1);
+        z = 2 + (b - j * (g.mz - 1));
+        if (h == 0) {
+            y = 2 + j * ty - 1;                   // last row of tile j - 1
+            if (j == 0 || y > g.my) return;
+        } else {
+            y = 2 + j * ty;                       // first row of tile j
+            if (j == ytiles) return;
+        }
     } else {
         b -= nA;
         const int j = b / (g.my - 1);
         y = 2 + (b - j * (g.my - 1));
-        const int z0 = 2 + (j >> 1) * zchunk;
-        const int z1 = min(z0 + zchunk, g.mz + 1);
-        z = (j & 1) ? z1 - 1 : z0;
-        if ((j & 1) && z == z0) return;           // a one-plane chunk: done once
         const int ry = (y - 2) % ty;
         if (ry == 0 || ry == ty - 1) return;      // a tile edge row: the first blocks have it
+        const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
+        if (h == 0) {
+            if (j == 0) return;
+            z = min(2 + j * zchunk - 1, g.mz);    // last plane of chunk j - 1
+            if (z == 2 + (j - 1) * zchunk) return;   // also its first: slot j - 1 has it
+        } else {
+            z = 2 + j * zchunk;                   // first plane of chunk j
+            if (j == zchunks) return;
+        }
     }
+    if (dn) z = g.mz + 2 - z;
     const int x0 = 2 + ((1 + y + z) & 1);         // x + y + z odd
-    for (int i = threadIdx.x; x0 + 2 * i <= g.mx; i += EDGE_THREADS) {
+    for (int i = lane; x0 + 2 * i <= g.mx; i += 128) {
         const int x = x0 + 2 * i;
         const long p = (long)z * g.sz + (long)y * g.sy + x;
         r[res3_at(g, z, y, x)] = resid3<T, T>(phi, rhs, cx, cy, cz, cc, p, g.sy, g.sz);

@@ -152,14 +152,16 @@ def test_fp32_cycle_coarse_program_register_runs(case, kw, reg):
 @pytest.mark.parametrize("rows,zchunk", [(0, 0), (5, 3), (14, 0), (6, 1)], ids=lambda v: str(v))
 @pytest.mark.parametrize("ms,pairs", [((256, 64, 64), 2), ((64, 32, 16), 2), ((32, 16, 64), 1)],
                          ids=lambda v: str(v))
-def test_fp32_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr):
+@pytest.mark.parametrize("zigzag", [1, 0])
+def test_fp32_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr, zigzag):
     """FP32 cycles whose last pre-smoothing pair sweep also stores the
     restriction's first-half residuals (k_resid_edges3 adds the tile edges),
     against the oracle run in float32; 256 wide with two pairs per thread is
     the 256^3 solve's sweep shape (rows of whole waves)."""
     case = ("3d_fr32", 3, ms, True, True, False)
     dp, op, phi, rhs = _setup(case, seed=41)
-    _tune(dp, FUSED_MIN_M=2, TWO_PASS_MIN=1, SWEEP_PAIRS=pairs, SWEEP_ROWS=rows, ZCHUNK=zchunk, FUSE_RESID=fr)
+    _tune(dp, FUSED_MIN_M=2, TWO_PASS_MIN=1, SWEEP_PAIRS=pairs, SWEEP_ROWS=rows, ZCHUNK=zchunk, FUSE_RESID=fr,
+          ZIGZAG=zigzag)
     for kw in (dict(), dict(gamma=2, nu2=2)):
         c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
         want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)

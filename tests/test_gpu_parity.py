@@ -168,23 +168,25 @@ def test_pair_sweep_zigzag_bitexact(ms, pairs, zchunk, zigzag):
 FR_SHAPES = [(0, 0), (4, 3), (5, 5), (6, 2), (14, 0), (14, 1)]
 
 
+@pytest.mark.parametrize("zigzag", [1, 0])
 @pytest.mark.parametrize("fr", [1, 0])
 @pytest.mark.parametrize("rows,zchunk", FR_SHAPES, ids=lambda v: str(v))
 @pytest.mark.parametrize("pairs", [1, 2])
 @pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 64), (64, 32, 16), (128, 32, 32)])
-def test_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr):
+def test_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr, zigzag):
     """The last pre-smoothing pair sweep storing the restriction's first-half
     residuals at its tiles' inner rows and planes, k_resid_edges3 the tile
     edge rows and chunk edge planes (JSV_TUNE_FUSE_RESID 1; 0: one residual
     pass): whole cycles give the oracle's bits.  Tile rows 4 .. 14, z-chunks
-    of 1 .. all planes; 128 wide: rows of whole waves."""
+    of 1 .. all planes; 128 wide: rows of whole waves; marching downward
+    (zig-zag after an upward sweep) and upward."""
     from jsolver_amd import _lib
     case = ("3d_fr", 3, ms, True, True, False)
     dp, op, phi0, rhs0 = _setup(case, seed=37)
     lib = _lib.load()
     for key, v in ((_lib.JSV_TUNE_FUSED_MIN_M, 2), (_lib.JSV_TUNE_TWO_PASS_MIN, 1),
                    (_lib.JSV_TUNE_SWEEP_PAIRS, pairs), (_lib.JSV_TUNE_SWEEP_ROWS, rows),
-                   (_lib.JSV_TUNE_ZCHUNK, zchunk), (_lib.JSV_TUNE_FUSE_RESID, fr)):
+                   (_lib.JSV_TUNE_ZCHUNK, zchunk), (_lib.JSV_TUNE_FUSE_RESID, fr), (_lib.JSV_TUNE_ZIGZAG, zigzag)):
         _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, key, v))
     for kw in (dict(), dict(nu1=2, nu2=1), dict(gamma=2, fcycle=True), dict(nu2=0)):
         c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
