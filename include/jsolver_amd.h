@@ -107,7 +107,12 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *   JSV_TUNE_SWEEP_ROWS   rows per workgroup tile of that kernel: 4, 6, 8, 16
  *                         (0 = 16 in FP32, 4 in FP64)
  *   JSV_TUNE_SWEEP_THREADS threads per workgroup of that kernel: 256 or 1024
- *                         (0 = 1024)
+ *                         (0 = 1024).  Pair kernel, FP32 two-pair sweeps: 0
+ *                         (default) the plain sweep and the one storing the
+ *                         first-half residuals in 10-row tiles of 768
+ *                         threads, the one with the prolongation in 14-row
+ *                         tiles of 1024; 768: all in 768; 1024: all but the
+ *                         residual-storing one in 1024
  *   JSV_TUNE_GRAPHS       1 (default): solves replay each cycle as a captured
  *                         hipGraph; 0: launch every kernel
  *   JSV_TUNE_COARSE_M     levels whose largest resolution is <= this (and

@@ -118,7 +118,8 @@ struct jsv_plan {
     int fused_min_m = 256;          // 3D levels with mz >= this use the fused out-of-place sweep
     int zchunk = 0;                 // planes per workgroup of the fused sweep (0: by level size)
     int sweep_rows = 0;             // rows per workgroup tile of the fused sweep (0: by precision)
-    int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024)
+    int sweep_threads = 0;          // threads per workgroup of the fused sweep (0: 1024; FP32 two-pair
+                                    // sweeps without the prolongation 768)
     int sweep_variant = 2;          // 1: thread per ring point; 2: thread per column pair, pipelined
     int sweep_pairs = 0;            // pair kernel: column pairs per thread (0: 2 in FP32, 1 in FP64)
     int n_cu = 256;                 // compute units of the device
@@ -434,25 +435,28 @@ int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zc
     }
 }
 
-// The pre-smoothing pair sweep that also stores the first-half residuals
-// (FR) with NT threads per workgroup: FP32 two-pair tiles of 10 rows take 768
-// threads, whose waves may hold 168 VGPRs (at 1024 threads and 14 rows that
-// variant spills).  Returns 1 (nothing launched) where the shape does not fit.
-template <typename T, int TY, int NHM, int NP, int NT>
-int launch_rbsweep3p_fr(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, FrGeo* fr)
+// A pair sweep with NT threads per workgroup: plain, with the prolongation
+// from pc, or (fr set) storing the first-half residuals (FR).  FP32 two-pair
+// tiles of 10 rows take 768 threads, whose waves may hold 168 VGPRs (at 1024
+// threads and 14 rows the FR variant spills); the FR sweep always runs so,
+// the others when JSV_TUNE_SWEEP_THREADS asks for 768.  Returns 1 (nothing
+// launched) where the shape does not fit.
+template <typename T, int TY, int NHM, int NP, int NT, bool PRO, bool FR>
+int launch_rbsweep3p_nt_v(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s, FrGeo* fr)
 {
     const Geo& g = H.g[l];
+    const Geo gc = PRO ? H.g[l + 1] : g;
     const int nhe = g.mx / 2;
-    const size_t lds = pair_sweep_lds(g, TY, sizeof(T), nullptr, NP, false, true, NT);
+    const size_t lds = pair_sweep_lds(g, TY, sizeof(T), PRO ? &gc : nullptr, NP, false, FR, NT);
     if ((g.nx + 1) / 2 > NHM || nhe % NP != 0 || (TY + 2) * (nhe / NP) > NT || lds > 160 * 1024 ||
-        (double)g.n * sizeof(T) >= 4294967296.0)
+        (double)g.n * sizeof(T) >= 4294967296.0 || (PRO && (double)gc.n * sizeof(T) >= 4294967296.0))
         return 1;
     auto no_scratch = [](const void* k) {
         hipFuncAttributes a;
         return hipFuncGetAttributes(&a, k) == hipSuccess && a.localSizeBytes == 0;
     };
-    const auto kwr = jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, true, false, true>;
-    const auto krt = jsv::k_rbsweep3p<T, TY, NT, NHM, false, NP, 1, false, false, true>;
+    const auto kwr = jsv::k_rbsweep3p<T, TY, NT, NHM, PRO, NP, 1, true, false, FR>;
+    const auto krt = jsv::k_rbsweep3p<T, TY, NT, NHM, PRO, NP, 1, false, false, FR>;
     static const bool wr_ok = no_scratch((const void*)kwr), rt_ok = no_scratch((const void*)krt);
     const bool wr = (nhe / NP) % 64 == 0 && wr_ok;
     if (!wr && !rt_ok) return 1;
@@ -465,19 +469,36 @@ int launch_rbsweep3p_fr(const jsv_plan* P, Hier<T>& H, int l, hipStream_t s, FrG
     if (zc > g.mz - 1) zc = g.mz - 1;
     const int zchunk = P->zchunk > 0 ? P->zchunk : (g.mz - 1 + zc - 1) / zc;
     const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
-    // zig-zag: the reverse of the level's previous pair sweep
+    // zig-zag: a sweep without the prolongation reverses the level's previous
+    // pair sweep; with it, the march is upward
     unsigned& last = P->zz_dn[sizeof(T) == 8 ? 0 : 1];
     const unsigned bit = 1u << l;
-    const int dn = (P->zigzag && (last & bit) == 0u) ? 1 : 0;
+    const int dn = (!PRO && P->zigzag && (last & bit) == 0u) ? 1 : 0;
     last = dn ? (last | bit) : (last & ~bit);
     hipLaunchKernelGGL(wr ? kwr : krt, dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds, s, H.phi[l], H.phi2[l],
-                       H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, (const T*)nullptr, g,
-                       P->act_ptr(), (double*)nullptr, dn, H.res[l]);
-    fr->ty = TY;
-    fr->zchunk = zchunk;
-    fr->ytiles = ytiles;
-    fr->dn = dn;
+                       H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, PRO ? pc : (const T*)nullptr,
+                       gc, P->act_ptr(), (double*)nullptr, dn, FR ? H.res[l] : (T*)nullptr);
+    if (FR) {
+        fr->ty = TY;
+        fr->zchunk = zchunk;
+        fr->ytiles = ytiles;
+        fr->dn = dn;
+    }
     return 0;
+}
+
+template <typename T, int TY, int NP, int NT>
+int launch_rbsweep3p_nt(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s, FrGeo* fr)
+{
+    const bool wide = (H.g[l].nx + 1) / 2 > 130;
+    if (fr != nullptr)
+        return wide ? launch_rbsweep3p_nt_v<T, TY, 258, NP, NT, false, true>(P, H, l, pc, s, fr)
+                    : launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, false, true>(P, H, l, pc, s, fr);
+    if (pc != nullptr)
+        return wide ? launch_rbsweep3p_nt_v<T, TY, 258, NP, NT, true, false>(P, H, l, pc, s, fr)
+                    : launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, true, false>(P, H, l, pc, s, fr);
+    return wide ? launch_rbsweep3p_nt_v<T, TY, 258, NP, NT, false, false>(P, H, l, pc, s, fr)
+                : launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, false, false>(P, H, l, pc, s, fr);
 }
 
 // Pairs per thread, rows and z-chunk of the pair sweep.  Two pairs per thread
@@ -497,12 +518,17 @@ int launch_rbsweep3p(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStrea
     // tuning forces
     int np = P->sweep_pairs > 0 ? P->sweep_pairs : ((sizeof(T) == 4 && g.mx >= 256) ? 2 : 1);
     if (np == 2 && !pair_sweep_rows_fit(g, 2, sizeof(T), gc, 2)) np = 1;
+    // FP32 two-pair sweeps in 10-row tiles of 768 threads: the one storing
+    // the first-half residuals always, the plain one by default (101 against
+    // 106 us at 256^3), the one with the prolongation when tuned so (129
+    // against 122 us); 1024 threads otherwise
     if constexpr (sizeof(T) == 4) {
-        if (fr != nullptr && np == 2 && pc == nullptr && res == nullptr && H.res[l] != nullptr && P->fuse_resid &&
-            (P->sweep_rows == 0 || P->sweep_rows == 10)) {
-            const int rc = (g.nx + 1) / 2 <= 130 ? launch_rbsweep3p_fr<T, 10, 130, 2, 768>(P, H, l, s, fr)
-                                                 : launch_rbsweep3p_fr<T, 10, 258, 2, 768>(P, H, l, s, fr);
-            if (rc == 0) return 0;
+        if (np == 2 && res == nullptr && (P->sweep_rows == 0 || P->sweep_rows == 10)) {
+            const bool frv = fr != nullptr && pc == nullptr && H.res[l] != nullptr && P->fuse_resid;
+            const bool plain768 = pc == nullptr && P->sweep_threads == 0;
+            if (frv || plain768 || P->sweep_threads == 768) {
+                if (launch_rbsweep3p_nt<T, 10, 2, 768>(P, H, l, pc, s, frv ? fr : nullptr) == 0) return 0;
+            }
         }
     }
     int rows = P->sweep_rows;
@@ -1728,7 +1754,8 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         return 0;
     }
     if (key == JSV_TUNE_SWEEP_THREADS) {
-        if (value != 0 && value != 256 && value != 1024) return fail("sweep threads must be 0, 256 or 1024");
+        if (value != 0 && value != 256 && value != 768 && value != 1024)
+            return fail("sweep threads must be 0, 256, 768 or 1024");
         P->sweep_threads = (int)value;
         return 0;
     }

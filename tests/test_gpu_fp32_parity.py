@@ -62,16 +62,19 @@ def test_fp32_smooth_bitexact(case):
         np.testing.assert_array_equal(dp.download(level, 0, prec=F32), want, err_msg=f"level {level}")
 
 
+@pytest.mark.parametrize("threads", [0, 1024])
 @pytest.mark.parametrize("zigzag", [0, 1])
 @pytest.mark.parametrize("pairs", [1, 2])
-@pytest.mark.parametrize("ms", [(32, 16, 16), (64, 32, 16)])
-def test_fp32_fused_pair_sweep_bitexact(ms, pairs, zigzag):
+@pytest.mark.parametrize("ms", [(32, 16, 16), (64, 32, 16), (256, 64, 64)])
+def test_fp32_fused_pair_sweep_bitexact(ms, pairs, zigzag, threads):
     """The fused out-of-place pair sweep (the 256^3 fine level's kernel),
     forced onto small levels; with zig-zag on, one of each level's two
-    sweeps marches downward."""
+    sweeps marches downward.  Two pairs per thread: 10-row tiles of 768
+    threads (default) or 14-row tiles of 1024 (256 wide: rows of whole
+    waves)."""
     case = ("3d", 3, ms, True, True, False)
     dp, op, _, _ = _setup(case, seed=5)
-    _tune(dp, FUSED_MIN_M=2, SWEEP_PAIRS=pairs, ZIGZAG=zigzag)
+    _tune(dp, FUSED_MIN_M=2, SWEEP_PAIRS=pairs, ZIGZAG=zigzag, SWEEP_THREADS=threads)
     for level in range(2):
         phi, rhs = _level_arrays(op, level, 20 + level)
         dp.upload(level, 0, phi, prec=F32)
@@ -172,14 +175,17 @@ def test_fp32_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr, z
             np.testing.assert_array_equal(dp.download(0, 0, prec=F32), want, err_msg=f"{kw} call {rep}")
 
 
+@pytest.mark.parametrize("threads", [0, 768, 1024])
 @pytest.mark.parametrize("fuse_prolong", [0, 2])
 @pytest.mark.parametrize("kw", [dict(), dict(gamma=2, nu2=2), dict(fcycle=True)], ids=["V", "W", "F"])
-def test_fp32_cycle_fused_sweep_bitexact(kw, fuse_prolong):
+def test_fp32_cycle_fused_sweep_bitexact(kw, fuse_prolong, threads):
     """A 3D FP32 cycle with the fine level on the fused pair sweep (two pairs
-    per thread, prolongation inside the post-smoothing sweep or separate)."""
+    per thread, prolongation inside the post-smoothing sweep or separate;
+    JSV_TUNE_SWEEP_THREADS 768: every two-pair sweep in 10-row tiles of 768
+    threads, 1024: in 14-row tiles of 1024 where that fits)."""
     case = ("3d", 3, (64, 32, 32), True, True, False)
     dp, op, phi, rhs = _setup(case, seed=9)
-    _tune(dp, FUSED_MIN_M=32, SWEEP_PAIRS=2, FUSE_PROLONG=fuse_prolong)
+    _tune(dp, FUSED_MIN_M=32, SWEEP_PAIRS=2, FUSE_PROLONG=fuse_prolong, SWEEP_THREADS=threads)
     c = gmg.Config(fixed=True, max_iters_outer=1, **kw)
     want = gmg.cycle(op, c, gmg.zero_rim(phi), rhs, c.fcycle)
     dp.upload(0, 0, gmg.zero_rim(phi), prec=F32)
