@@ -438,9 +438,9 @@ int launch_rbsweep3p_rows(const jsv_plan* P, Hier<T>& H, int l, int rows, int zc
 // A pair sweep with NT threads per workgroup: plain, with the prolongation
 // from pc, or (fr set) storing the first-half residuals (FR).  FP32 two-pair
 // tiles of 10 rows take 768 threads, whose waves may hold 168 VGPRs (at 1024
-// threads and 14 rows the FR variant spills); the FR sweep always runs so,
-// the others when JSV_TUNE_SWEEP_THREADS asks for 768.  Returns 1 (nothing
-// launched) where the shape does not fit.
+// threads and 14 rows the FR variant spills).  Returns 1 (nothing launched)
+// where the shape does not fit; launch_rbsweep3p picks the variants that
+// run so.
 template <typename T, int TY, int NHM, int NP, int NT, bool PRO, bool FR>
 int launch_rbsweep3p_nt_v(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s, FrGeo* fr)
 {
