@@ -487,18 +487,15 @@ int launch_rbsweep3p_nt_v(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hip
     return 0;
 }
 
+// Rows of at most 260 points only: a wider row needs more than NT / (TY + 2)
+// threads at two pairs per thread for these shapes.
 template <typename T, int TY, int NP, int NT>
 int launch_rbsweep3p_nt(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hipStream_t s, FrGeo* fr)
 {
-    const bool wide = (H.g[l].nx + 1) / 2 > 130;
-    if (fr != nullptr)
-        return wide ? launch_rbsweep3p_nt_v<T, TY, 258, NP, NT, false, true>(P, H, l, pc, s, fr)
-                    : launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, false, true>(P, H, l, pc, s, fr);
-    if (pc != nullptr)
-        return wide ? launch_rbsweep3p_nt_v<T, TY, 258, NP, NT, true, false>(P, H, l, pc, s, fr)
-                    : launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, true, false>(P, H, l, pc, s, fr);
-    return wide ? launch_rbsweep3p_nt_v<T, TY, 258, NP, NT, false, false>(P, H, l, pc, s, fr)
-                : launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, false, false>(P, H, l, pc, s, fr);
+    if ((H.g[l].nx + 1) / 2 > 130) return 1;
+    if (fr != nullptr) return launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, false, true>(P, H, l, pc, s, fr);
+    if (pc != nullptr) return launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, true, false>(P, H, l, pc, s, fr);
+    return launch_rbsweep3p_nt_v<T, TY, 130, NP, NT, false, false>(P, H, l, pc, s, fr);
 }
 
 // Pairs per thread, rows and z-chunk of the pair sweep.  Two pairs per thread
