@@ -719,15 +719,25 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         if constexpr (PRO) {
             T* dst = P + (z % 3) * PR * nh;
             cpr_ok = 0;
-            // not unrolled: one element's coarse reads live at a time
-#pragma unroll 1
-            for (int j = 0; j < PMAX; ++j) {
+            auto one = [&](int j) {
                 const int i = tid + j * NT;
                 T c;
                 if (i < PR * nh && corr_of(z, i, c)) {
                     dst[i] = c;
                     cpr_ok |= 1u << j;
                 }
+            };
+            if constexpr (sizeof(T) == 8 && WR) {
+                // FP64 with rows of whole waves (two elements per thread): the
+                // elements' coarse reads overlap (224 -> 220 us at 256^3, no
+                // spills; the per-thread-pick variant would spill)
+#pragma unroll
+                for (int j = 0; j < PMAX; ++j) one(j);
+            } else {
+                // FP32: one element's coarse reads live at a time (unrolled,
+                // the two-pair variants spill)
+#pragma unroll 1
+                for (int j = 0; j < PMAX; ++j) one(j);
             }
         }
     };
