@@ -735,7 +735,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                 for (int j = 0; j < PMAX; ++j) one(j);
             } else {
                 // FP32: one element's coarse reads live at a time (unrolled,
-                // the two-pair variants spill)
+                // the two-pair variants spill; unrolled by two, no faster)
 #pragma unroll 1
                 for (int j = 0; j < PMAX; ++j) one(j);
             }
