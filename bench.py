@@ -83,7 +83,8 @@ def roofline(a, grids, m):
                 roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": _traffic(),
                         "kernel": "k_rbsweep3p<float>: one fused out-of-place RBGS sweep (red + black "
-                                  "halves, one launch), fine level",
+                                  "halves, one launch; two column pairs per thread in 10-row tiles of "
+                                  "768 threads), fine level",
                         "algorithmic_bytes_per_sweep": 7 * w * n_unk,
                         "us_per_sweep": round(t_sweep * 1e6, 2)}
     log("per-level sweeps: " + ", ".join(f"{p['prec']}/L{p['level']}:{p['GBps']}GB/s" for p in per_level))
