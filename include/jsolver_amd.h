@@ -164,7 +164,8 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         through at most three levels below it; 1: the
  *                         former only; 0: in LDS
  *   JSV_TUNE_ZIGZAG       1 (default): a pair sweep without the prolongation
- *                         or the residual sum marches its level's planes in
+ *                         or the residual sum (with or without the stored
+ *                         first-half residuals) marches its level's planes in
  *                         the opposite order of that level's previous pair
  *                         sweep (the Infinity Cache still holds the planes
  *                         the previous sweep touched last); 0: always upward
