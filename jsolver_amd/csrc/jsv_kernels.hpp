@@ -857,15 +857,26 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // downward march the stored plane below is the march's next one: the z
     // term taken at step z + 2 comes first, and the product with the march's
     // previous plane waits one more step.
-    T t0h[NP], pmh[NP];                         // cc * phi and the z - 1 product of plane zs - 1's red points
+    //
+    // Where a row of pairs lies within one wave (xs), the x terms come from
+    // this thread's black values and the neighbouring lanes' at the end of
+    // step z + 1, and the y terms after the first barrier of step z + 2 (the
+    // y products of plane z wait in Bu, Bd until that step's black half), so
+    // no third barrier is needed.  Otherwise (a row over several waves) the
+    // four in-plane terms are taken after a third barrier of step z + 1, the
+    // x neighbours from Bk.
+    T t0h[NP];                                  // cc * phi of plane zs - 1's red points
+    T carry[NP];                                // xs: plane zs - 2's x terms done; else plane zs - 1's z - 1 product
     T cxe = (T)0;                               // cx of plane zs - 1's red point at the thread's edge
 #pragma unroll
     for (int pp = 0; pp < NP; ++pp) {
         t0h[pp] = (T)0;
-        pmh[pp] = (T)0;
+        carry[pp] = (T)0;
     }
-    // 0, 1: red rhs by plane parity; 2: plane zs - 2's partial residual;
-    // 3 (downward march): plane zs - 2's product with plane zs - 3
+    const bool xs = gpr <= 64;                  // rows of pairs never straddle a wave
+    // 0, 1: red rhs by plane parity; xs: 2, 3: the z - 1 product by plane
+    // parity; otherwise 2: plane zs - 2's partial residual, 3 (downward
+    // march): plane zs - 2's z - 1 product
     auto HF = [&](int j, int pp) -> T& { return Hf[(j * NP + pp) * NT + tid]; };
     const bool fr_row = FR && inner && r >= 2 && r <= TY - 1;
     auto fr_plane = [&](int u) { return u >= z0 + 1 && u <= z1 - 2; };
@@ -898,6 +909,26 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         __syncthreads();
         // ---- red on plane zs (x + y + z odd); pair pp covers x = 2(k+pp), 2(k+pp)+1
         const int e = decltype(ec)::at(y, zs);  // red element within a pair: 0 when y + zs is odd
+        // FR, xs: plane zs - 2's red points (this plane's red element) take
+        // their y terms, and on an upward march the z - 1 term
+        T ptl[NP];
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp) ptl[pp] = (T)0;
+        if constexpr (FR) {
+            if (xs && fr_row && fr_plane(zs - 2)) {
+#pragma unroll
+                for (int pp = 0; pp < NP; ++pp) {
+                    const int x = 2 * (k + pp) + e;
+                    if (x <= g.mx) {
+                        const int row = (r - 2) * nh + (x >> 1);
+                        T rr = carry[pp] - Bu[row];
+                        rr = rr - Bd[row];
+                        if (!dn) rr = rr - HF(2 + (zs & 1), pp);
+                        ptl[pp] = rr;
+                    }
+                }
+            }
+        }
         // element of pair pp: e ? odd : even (selects between constant indices;
         // a runtime index would move the vectors to scratch memory)
 #define JSV_EL(vec, pp, odd) ((odd) ? (vec).v[2 * (pp) + 1] : (vec).v[2 * (pp)])
@@ -947,9 +978,12 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         __syncthreads();
         // ---- black on plane zb = zs - 1 at x = xr; write both colours of the pairs
         const int zb = zs - 1;
-        T pmn[NP];                              // FR
+        T pmn[NP], bk[NP];                      // FR
 #pragma unroll
-        for (int pp = 0; pp < NP; ++pp) pmn[pp] = (T)0;
+        for (int pp = 0; pp < NP; ++pp) {
+            pmn[pp] = (T)0;
+            bk[pp] = (T)0;
+        }
         if (inner && zb >= z0 && zb < z1) {
             T* orow = pout + ((long)pl(zb) * sz + (long)y * sy);
             // both colours of the thread's pairs leave as one vector store; a
@@ -990,19 +1024,23 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
                     // this black row is y - y0 = r - 1: row r - 2 of Bk, the row
                     // above's Bu, the row below's Bd
                     const int col = k + pp;
-                    if (r >= 2 && r <= TY - 1) Bk[(r - 2) * nh + col] = blk;
+                    if (!xs && r >= 2 && r <= TY - 1) Bk[(r - 2) * nh + col] = blk;
                     if (r <= TY - 2) Bu[(r - 1) * nh + col] = H(pp, 2) * blk;
                     if (r >= 3) Bd[(r - 3) * nh + col] = H(pp, 3) * blk;
+                    bk[pp] = blk;
                     // the z - 1 product of plane zs's red point (same x)
-                    pmn[pp] = JSV_EL(czp, pp, e) * blk;
+                    const T pm = JSV_EL(czp, pp, e) * blk;
                     // plane zs - 2's red point (same x): its z + 1 term (cz of that
                     // point is held as this black point's z - 1 face) and rhs
                     if (fr_row && fr_plane(zs - 2) && xr <= g.mx) {
-                        T rr = HF(2, pp) - H(pp, 1) * blk;
-                        if (dn) rr = rr - HF(3, pp);
+                        T rr = (xs ? ptl[pp] : HF(2, pp)) - H(pp, 1) * blk;
+                        if (dn) rr = rr - HF(xs ? 2 + (zs & 1) : 3, pp);
                         rr = rr + HF(zs & 1, pp);
                         rfo[res3_at(g, pl(zs - 2), y, xr)] = rr;
                     }
+                    // (xs: the slot of plane zs - 2's product, read just above)
+                    if (xs) HF(2 + (zs & 1), pp) = pm;
+                    else pmn[pp] = pm;
                 }
                 const T rp = ((xr ^ 1) <= g.mx) ? red_prev[pp] : (T)0;
                 ov.v[2 * pp] = e ? rp : blk;
@@ -1016,33 +1054,64 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
             for (int pp = 0; pp < NP; ++pp) Bd[(r - 3) * nh + k + pp] = H(pp, 3) * (T)0;
         }
         if constexpr (FR) {
-            __syncthreads();                    // every row's black values of plane zs - 1 are in Bk, Bu, Bd
-            // plane zs - 1's red points at x = 2(k + pp) + 1 - e: cc * phi, the
-            // in-plane terms, the z - 1 term.  Their cx faces are held by this
-            // thread's black halves of the same plane (cxe at its edge).
-            if (fr_row && fr_plane(zs - 1)) {
+            // plane zs - 1's red points at x = 2(k + pp) + 1 - e.  Their cx faces
+            // are held by this thread's black halves of the same plane (cxe at
+            // its edge).
+            auto faces = [&](int pp, T& fxm, T& fxp) {
+                if (e == 1) {
+                    fxp = H(pp, 5);
+                    fxm = pp > 0 ? H(pp - 1, 4) : cxe;
+                } else {
+                    fxm = H(pp, 4);
+                    fxp = pp + 1 < NP ? H(pp + 1, 5) : cxe;
+                }
+            };
+            if (xs) {
+                // cc * phi and the x terms: the x neighbours are this row's black
+                // values of the plane, this thread's or the lanes' beside it (x =
+                // 1 and x = mx + 1 are boundary nodes)
+                const T bl = __shfl_up(bk[NP - 1], 1, 64), br = __shfl_down(bk[0], 1, 64);
+                if (fr_row && fr_plane(zs - 1)) {
 #pragma unroll
-                for (int pp = 0; pp < NP; ++pp) {
-                    const int x = 2 * (k + pp) + 1 - e;
-                    if (x <= g.mx) {
-                        T fxm, fxp;
-                        if (e == 1) {
-                            fxp = H(pp, 5);
-                            fxm = pp > 0 ? H(pp - 1, 4) : cxe;
-                        } else {
-                            fxm = H(pp, 4);
-                            fxp = pp + 1 < NP ? H(pp + 1, 5) : cxe;
+                    for (int pp = 0; pp < NP; ++pp) {
+                        const int x = 2 * (k + pp) + 1 - e;
+                        if (x <= g.mx) {
+                            T fxm, fxp, bxm, bxp;
+                            faces(pp, fxm, fxp);
+                            if (e == 1) {               // x + 1 is this pair's black point
+                                bxp = bk[pp];
+                                bxm = pp > 0 ? bk[pp - 1] : (k == 1 ? (T)0 : bl);
+                            } else {                    // x - 1 is this pair's black point
+                                bxm = bk[pp];
+                                bxp = pp + 1 < NP ? bk[pp + 1] : br;
+                            }
+                            T rr = t0h[pp] - fxm * bxm;
+                            rr = rr - fxp * bxp;
+                            carry[pp] = rr;
                         }
-                        const int row = (r - 2) * nh;
-                        const T bxm = Bk[row + ((x - 1) >> 1)], bxp = Bk[row + ((x + 1) >> 1)];
-                        const T pym = Bu[row + (x >> 1)], pyp = Bd[row + (x >> 1)];
-                        T rr = t0h[pp] - fxm * bxm;
-                        rr = rr - fxp * bxp;
-                        rr = rr - pym;
-                        rr = rr - pyp;
-                        if (dn) HF(3, pp) = pmh[pp];
-                        else rr = rr - pmh[pp];
-                        HF(2, pp) = rr;
+                    }
+                }
+            } else {
+                __syncthreads();                // every row's black values of plane zs - 1 are in Bk, Bu, Bd
+                // cc * phi, the in-plane terms, the z - 1 term
+                if (fr_row && fr_plane(zs - 1)) {
+#pragma unroll
+                    for (int pp = 0; pp < NP; ++pp) {
+                        const int x = 2 * (k + pp) + 1 - e;
+                        if (x <= g.mx) {
+                            T fxm, fxp;
+                            faces(pp, fxm, fxp);
+                            const int row = (r - 2) * nh;
+                            const T bxm = Bk[row + ((x - 1) >> 1)], bxp = Bk[row + ((x + 1) >> 1)];
+                            const T pym = Bu[row + (x >> 1)], pyp = Bd[row + (x >> 1)];
+                            T rr = t0h[pp] - fxm * bxm;
+                            rr = rr - fxp * bxp;
+                            rr = rr - pym;
+                            rr = rr - pyp;
+                            if (dn) HF(3, pp) = carry[pp];
+                            else rr = rr - carry[pp];
+                            HF(2, pp) = rr;
+                        }
                     }
                 }
             }
@@ -1063,7 +1132,7 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
             if constexpr (FR) {
                 HF(zs & 1, pp) = JSV_EL(cur.rh, pp, e);
                 t0h[pp] = JSV_EL(cur.dg, pp, e) * red[pp];
-                pmh[pp] = pmn[pp];
+                if (!xs) carry[pp] = pmn[pp];
             }
             red_prev[pp] = red[pp];
         }
