@@ -846,10 +846,10 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // FR: the residual rhs - L phi at the red points (x + y + z odd) of the
     // swept level, in resid3's term order, written to rfo at res3_at.  The
     // residual of plane z needs the black values of planes z - 1, z and
-    // z + 1, so it is formed over two steps: cc * phi and the z - 1 product
-    // when the red value is computed (step z), the four in-plane terms and
-    // the z - 1 term after the black half of plane z (step z + 1), the z + 1
-    // term and rhs after the black half of plane z + 1 (step z + 2).  Only
+    // z + 1, so it is formed over three steps: cc * phi and the z - 1
+    // product when the red value is computed (step z), the four in-plane
+    // terms and the z - 1 term after the black half of plane z (step z + 1),
+    // the z + 1 term and rhs after the black half of plane z + 1 (step z + 2).  Only
     // the tile's rows y0 + 1 .. y0 + TY - 2 and planes z0 + 1 .. z1 - 2 are
     // done here: the other rows' and planes' neighbours belong to other
     // tiles (k_resid_edges3 covers those points).  Products are formed
