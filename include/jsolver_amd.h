@@ -163,12 +163,17 @@ int jsv_plan_device_bytes(const jsv_plan *plan, int64_t *bytes);
  *                         at most 256 points (a 2 x 2 block per lane) down
  *                         through at most three levels below it; 1: the
  *                         former only; 0: in LDS
- *   JSV_TUNE_ZIGZAG       1 (default): a pair sweep without the prolongation
- *                         or the residual sum (with or without the stored
- *                         first-half residuals) marches its level's planes in
- *                         the opposite order of that level's previous pair
- *                         sweep (the Infinity Cache still holds the planes
- *                         the previous sweep touched last); 0: always upward
+ *   JSV_TUNE_ZIGZAG       2 (default): every pair sweep except the one with
+ *                         the residual sum marches each z-chunk in the
+ *                         opposite direction of that chunk's previous pair
+ *                         sweep (the Infinity Cache still holds the planes it
+ *                         touched last), and neighbouring chunks march in
+ *                         opposite directions, so both reach their shared
+ *                         boundary planes at the same time; 1: the whole
+ *                         level marches opposite to its previous pair sweep,
+ *                         for sweeps without the prolongation or the residual
+ *                         sum (with or without the stored first-half
+ *                         residuals); 0: always upward
  *   JSV_TUNE_FUSE_RESID   1 (default): the last pre-smoothing pair sweep of a
  *                         3D level with the two-pass restriction also stores
  *                         the residuals that restriction reads, at the red

@@ -123,7 +123,8 @@ struct jsv_plan {
     int sweep_variant = 2;          // 1: thread per ring point; 2: thread per column pair, pipelined
     int sweep_pairs = 0;            // pair kernel: column pairs per thread (0: 2 in FP32, 1 in FP64)
     int n_cu = 256;                 // compute units of the device
-    int zigzag = 1;                 // a plain pair sweep marches opposite to its level's previous pair sweep
+    int zigzag = 2;                 // pair sweeps march opposite to their level's previous pair sweep (2: in
+                                    // z-chunks of alternating direction; 1: the whole level, plain sweeps only)
     int fuse_resid = 1;             // the last pre-smoothing pair sweep stores the restriction's residuals
     mutable unsigned zz_dn[2] = {0u, 0u};   // FP64 / FP32: bit l = level l's last pair sweep marched downward
     int fuse_prolong = 2;           // 3D: post-smoothing pair sweeps apply the prolongation (1: FP32 only, 2: both)
@@ -358,16 +359,19 @@ int launch_rbsweep3p_t(const jsv_plan* P, Hier<T>& H, int l, int zchunk, const T
     const bool wr = ((g.mx / 2) / NP) % 64 == 0 && (pc != nullptr ? wr_pro : wr_plain);
     auto go = [&](auto kern, const T* pcv, size_t lds_k = 0, double* rp = nullptr, T* rfo = nullptr) {
         // zig-zag: a plain sweep (with or without the first-half residuals)
-        // reverses its level's previous pair sweep (the variants with the
-        // prolongation or the residual sum march upward)
+        // reverses its level's previous pair sweep; with chunks of alternating
+        // direction (zig-zag 2) so does the sweep with the prolongation; the
+        // sweep with the residual sum marches upward
         unsigned& last = P->zz_dn[sizeof(T) == 8 ? 0 : 1];
         const unsigned bit = 1u << l;
-        const int dn = (P->zigzag && pcv == nullptr && rp == nullptr && (last & bit) == 0u) ? 1 : 0;
+        const bool rev = rp == nullptr && (pcv == nullptr ? P->zigzag != 0 : P->zigzag == 2);
+        const int dn = (rev && (last & bit) == 0u) ? 1 : 0;
         last = dn ? (last | bit) : (last & ~bit);
-        if (fr != nullptr) fr->dn = dn;
+        const int alt = P->zigzag == 2 && rp == nullptr ? 2 : 0;   // chunks of alternating direction
+        if (fr != nullptr) fr->dn = alt ? 0 : dn;
         hipLaunchKernelGGL(kern, grid, dim3(NT), lds_k > 0 ? lds_k : lds, s, H.phi[l], H.phi2[l], H.rhs[l],
-                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp, dn,
-                           rfo);
+                           H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, pcv, gc, P->act_ptr(), rp,
+                           dn | alt, rfo);
     };
     if (nres != nullptr) *nres = 0;
     if constexpr (TY >= 4) {
@@ -470,19 +474,22 @@ int launch_rbsweep3p_nt_v(const jsv_plan* P, Hier<T>& H, int l, const T* pc, hip
     const int zchunk = P->zchunk > 0 ? P->zchunk : (g.mz - 1 + zc - 1) / zc;
     const int zchunks = (g.mz - 1 + zchunk - 1) / zchunk;
     // zig-zag: a sweep without the prolongation reverses the level's previous
-    // pair sweep; with it, the march is upward
+    // pair sweep; with it, only with chunks of alternating direction
+    // (zig-zag 2), and otherwise the march is upward
     unsigned& last = P->zz_dn[sizeof(T) == 8 ? 0 : 1];
     const unsigned bit = 1u << l;
-    const int dn = (!PRO && P->zigzag && (last & bit) == 0u) ? 1 : 0;
+    const bool rev = PRO ? P->zigzag == 2 : P->zigzag != 0;
+    const int dn = (rev && (last & bit) == 0u) ? 1 : 0;
     last = dn ? (last | bit) : (last & ~bit);
+    const int alt = P->zigzag == 2 ? 2 : 0;   // chunks of alternating direction
     hipLaunchKernelGGL(wr ? kwr : krt, dim3((unsigned)(ytiles * zchunks)), dim3(NT), lds, s, H.phi[l], H.phi2[l],
                        H.rhs[l], H.cx[l], H.cy[l], H.cz[l], H.cc[l], g, zchunk, ytiles, PRO ? pc : (const T*)nullptr,
-                       gc, P->act_ptr(), (double*)nullptr, dn, FR ? H.res[l] : (T*)nullptr);
+                       gc, P->act_ptr(), (double*)nullptr, dn | alt, FR ? H.res[l] : (T*)nullptr);
     if (FR) {
         fr->ty = TY;
         fr->zchunk = zchunk;
         fr->ytiles = ytiles;
-        fr->dn = dn;
+        fr->dn = alt ? 0 : dn;   // alternating chunks keep the upward march's chunk bounds
     }
     return 0;
 }
@@ -1716,7 +1723,8 @@ int jsv_plan_set_tuning(jsv_plan* P, int key, int64_t value)
         return 0;
     }
     if (key == JSV_TUNE_ZIGZAG) {
-        P->zigzag = value != 0 ? 1 : 0;
+        if (value > 2) return fail("JSV_TUNE_ZIGZAG takes 0, 1 or 2, not %ld", (long)value);
+        P->zigzag = (int)value;
         return 0;
     }
     if (key == JSV_TUNE_FUSE_RESID) {

@@ -611,9 +611,24 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int y0 = 2 + (tile % ytiles) * TY;
-    const int z0 = 2 + (tile / ytiles) * zchunk;
+    const int zci = tile / ytiles;              // z-chunk index
+    // dn (below): this workgroup marches downward.  dn_in bit 0: the
+    // direction; bit 1: chunks alternate, chunk zci marching in direction
+    // bit 0 ^ (zci & 1), and chunk bounds are those of the upward march in
+    // stored planes whatever the direction, so neighbouring chunks reach
+    // their shared boundary planes at the same time (both at the start or
+    // both at the end of the march)
+    const bool zalt = (dn_in & 2) != 0;
+    const int dn = RES ? 0 : ((dn_in & 1) ^ (zalt ? (zci & 1) : 0));
+    int z0 = 2 + zci * zchunk;
     int z1 = z0 + zchunk;
     if (z1 > g.mz + 1) z1 = g.mz + 1;
+    if (zalt && dn) {
+        // stored planes z0 .. z1 - 1 are march planes mz + 3 - z1 .. mz + 2 - z0
+        const int u0 = g.mz + 3 - z1;
+        z1 = g.mz + 3 - z0;
+        z0 = u0;
+    }
     const long sy = g.sy, sz = g.sz;
 
     for (int i = tid; i < 3 * RR * nh; i += NT) R[i] = (T)0;
@@ -643,11 +658,11 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // above march plane u is the stored cz of plane pl(u + 1) (clamped for the
     // prefetch past the last plane, whose values are never used).  The two z
     // terms of a stencil sum swap places, and that first sum is commutative,
-    // so the bits are those of the upward march.  The variants with the
-    // prolongation or the residual sum (whose term orders are not symmetric in
-    // z) always march upward; the first-half residuals (FR) take their two z
-    // terms in stored-plane order.
-    const int dn = (PRO || RES) ? 0 : dn_in;
+    // so the bits are those of the upward march.  The variant with the
+    // residual sum (whose term order is not symmetric in z) always marches
+    // upward; the first-half residuals (FR) take their two z terms in
+    // stored-plane order, and the prolongation's corrections (PRO) are
+    // evaluated at stored planes from a ring of stored coarse planes.
     auto pl = [&](int u) -> int { return dn ? g.nz - 1 - u : u; };
     auto plz = [&](int u) -> int { return dn ? max(g.nz - 2 - u, 0) : u; };
     auto at = [](const T* base, unsigned boff) -> T {
@@ -689,7 +704,9 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // PRO: the prolongation phi += P e (solver_3d.py:654-677) is applied to
     // phi_in's black values as they enter the ring (with nu2 > 0 the red
     // points' corrections are never read).  e comes from a ring of three
-    // coarse planes; fine plane f needs coarse planes (f+1)>>1 .. (f+2)>>1.
+    // coarse planes, indexed by stored plane; stored fine plane f needs coarse
+    // planes (f+1)>>1 .. (f+2)>>1, so the window moves up the coarse level
+    // in an upward march and down it in a downward one.
     constexpr int cnx = NHMAX + 1;              // coarse ring row width (>= gc.nx)
     const int cr0 = (y0 - 2) >> 1;              // first coarse row of the ring
     auto cval = [&](int I, int J, int K) -> T { return Cr[((I % 3) * CRR + (J - cr0)) * cnx + K]; };
@@ -698,9 +715,9 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         const int rr_ = i / nh, kk = i - rr_ * nh;
         const int yy = y0 - 2 + rr_;
         const int x = 2 * kk + ((yy + z) & 1);
-        // a black point: x + yy + z is even
+        // a black point: x + yy + z is even (and x + yy + pl(z): mz + 2 is even)
         if (x < 2 || x > g.mx || yy < 2 || yy > g.my || z < 2 || z > g.mz) return false;
-        corr = prolong3_corr_even<T>(cval, x, yy, z);
+        corr = prolong3_corr_even<T>(cval, x, yy, pl(z));
         return true;
     };
     auto corrected = [&](int z, int i, T v) -> T {
@@ -776,7 +793,8 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
             if (i < CRR * cnx) dst[i] = ((cpre_ok >> j) & 1u) ? cpre[j] : (T)0;
         }
     };
-    auto cmax = [](int f) { return (f + 2) >> 1; };
+    auto cmax = [](int f) { return (f + 2) >> 1; };   // highest coarse plane of stored fine plane f
+    auto cmin = [](int f) { return (f + 1) >> 1; };   // lowest (>= 0 for f >= -1)
 
     auto commit = [&](int z) {
         T* dst = P + (z % 3) * PR * nh;
@@ -824,10 +842,17 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
     // that plane is refilled by the next step's commit before any barrier)
     auto Pbv = [&](int pp) -> T& { return Pb[pp * NT + tid]; };
 
-    int chave = 0;                              // PRO: highest coarse plane in the ring
+    // PRO: the ring's newest coarse plane (upward: its highest, downward: its
+    // lowest); the march planes z0 - 2 .. z0 + 1 need three
+    int chave = 0;
     if constexpr (PRO) {
-        chave = cmax(z0 + 1);
-        for (int I = (z0 - 1) >> 1; I <= chave; ++I) cload_plane(I);
+        if (dn) {
+            chave = cmin(pl(z0 + 1));
+            for (int I = chave; I <= cmax(pl(z0 - 2)); ++I) cload_plane(I);
+        } else {
+            chave = cmax(z0 + 1);
+            for (int I = (z0 - 1) >> 1; I <= chave; ++I) cload_plane(I);
+        }
         __syncthreads();
     }
     load_plane(z0 - 2, corrected);
@@ -893,12 +918,21 @@ k_rbsweep3p(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict
         if constexpr (PRO) {
             // coarse plane for the next step's commit (issued one step ago);
             // it replaces a plane no commit of this step reads
-            const int want = cmax(zs + 2);
-            if (want > chave) {
-                ccommit(want);
-                chave = want;
+            if (dn) {
+                const int want = cmin(pl(zs + 2));   // >= 0: pl(zs + 2) >= mz - z1 >= -1
+                if (want < chave) {
+                    ccommit(want);
+                    chave = want;
+                }
+                cissue(cmin(pl(zs + 3)));   // a plane below 0 is not loaded, nor committed
+            } else {
+                const int want = cmax(zs + 2);
+                if (want > chave) {
+                    ccommit(want);
+                    chave = want;
+                }
+                cissue(cmax(zs + 3));
             }
-            cissue(cmax(zs + 3));
         }
         issue(zs + 2);
         load(zs + 1, nxt);

@@ -63,7 +63,7 @@ def test_fp32_smooth_bitexact(case):
 
 
 @pytest.mark.parametrize("threads", [0, 1024])
-@pytest.mark.parametrize("zigzag", [0, 1])
+@pytest.mark.parametrize("zigzag", [0, 1, 2])
 @pytest.mark.parametrize("pairs", [1, 2])
 @pytest.mark.parametrize("ms", [(32, 16, 16), (64, 32, 16), (256, 64, 64)])
 def test_fp32_fused_pair_sweep_bitexact(ms, pairs, zigzag, threads):
@@ -155,7 +155,7 @@ def test_fp32_cycle_coarse_program_register_runs(case, kw, reg):
 @pytest.mark.parametrize("rows,zchunk", [(0, 0), (5, 3), (14, 0), (6, 1)], ids=lambda v: str(v))
 @pytest.mark.parametrize("ms,pairs", [((256, 64, 64), 2), ((64, 32, 16), 2), ((32, 16, 64), 1)],
                          ids=lambda v: str(v))
-@pytest.mark.parametrize("zigzag", [1, 0])
+@pytest.mark.parametrize("zigzag", [1, 0, 2])
 def test_fp32_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr, zigzag):
     """FP32 cycles whose last pre-smoothing pair sweep also stores the
     restriction's first-half residuals (k_resid_edges3 adds the tile edges),

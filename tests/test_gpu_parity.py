@@ -134,14 +134,15 @@ def test_fused_sweep_bitexact(ms, zchunk, variant, rows, pairs):
         np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle))
 
 
-@pytest.mark.parametrize("zigzag", [0, 1])
+@pytest.mark.parametrize("zigzag", [0, 1, 2])
 @pytest.mark.parametrize("zchunk", [1, 3, 0])
 @pytest.mark.parametrize("pairs", [1, 2])
 @pytest.mark.parametrize("ms", [(16, 32, 32), (32, 16, 64), (8, 8, 8)])
 def test_pair_sweep_zigzag_bitexact(ms, pairs, zchunk, zigzag):
     """Pair sweeps that march downward (every other plain sweep of a level
-    with zig-zag on) give the oracle's bits; so do 1, 2 and 3 consecutive
-    sweeps, which start in either direction, and the cycles around them."""
+    with zig-zag on; zig-zag 2: every other z-chunk of each sweep) give the
+    oracle's bits; so do 1, 2 and 3 consecutive sweeps, which start in either
+    direction, and the cycles around them."""
     from jsolver_amd import _lib
     case = ("3d_zigzag", 3, ms, True, True, False)
     dp, op, phi0, rhs0 = _setup(case, seed=17)
@@ -168,7 +169,7 @@ def test_pair_sweep_zigzag_bitexact(ms, pairs, zchunk, zigzag):
 FR_SHAPES = [(0, 0), (4, 3), (5, 5), (6, 2), (14, 0), (14, 1)]
 
 
-@pytest.mark.parametrize("zigzag", [1, 0])
+@pytest.mark.parametrize("zigzag", [1, 0, 2])
 @pytest.mark.parametrize("fr", [1, 0])
 @pytest.mark.parametrize("rows,zchunk", FR_SHAPES, ids=lambda v: str(v))
 @pytest.mark.parametrize("pairs", [1, 2])
@@ -179,7 +180,8 @@ def test_fused_first_half_residuals_bitexact(ms, pairs, rows, zchunk, fr, zigzag
     edge rows and chunk edge planes (JSV_TUNE_FUSE_RESID 1; 0: one residual
     pass): whole cycles give the oracle's bits.  Tile rows 4 .. 14, z-chunks
     of 1 .. all planes; 128 wide: rows of whole waves; marching downward
-    (zig-zag after an upward sweep) and upward."""
+    (zig-zag after an upward sweep), upward, and in chunks of alternating
+    direction (zig-zag 2)."""
     from jsolver_amd import _lib
     case = ("3d_fr", 3, ms, True, True, False)
     dp, op, phi0, rhs0 = _setup(case, seed=37)
