@@ -473,6 +473,23 @@ def test_coarse_tuning_limits():
     assert lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_COARSE_REG, 3) != 0
 
 
+def test_zigzag_tuning_limit():
+    """JSV_TUNE_ZIGZAG takes 0, 1 or 2; a larger value is an error, and the
+    plan keeps working with its previous setting."""
+    from jsolver_amd import _lib
+    lib = _lib.load()
+    case = ("3d_zigzag", 3, (16, 32, 32), True, True, False)
+    dp, op, phi0, rhs0 = _setup(case, seed=3)
+    assert lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_ZIGZAG, 3) != 0
+    assert b"JSV_TUNE_ZIGZAG" in lib.jsv_last_error()
+    _lib.check(lib.jsv_plan_set_tuning(dp.plan.h, _lib.JSV_TUNE_FUSED_MIN_M, 2))
+    dp.upload(0, 0, phi0)
+    dp.upload(0, 1, rhs0)
+    dp.cycle(cfg())
+    c = gmg.Config(fixed=True, max_iters_outer=1)
+    np.testing.assert_array_equal(dp.download(0, 0), gmg.cycle(op, c, gmg.zero_rim(phi0), rhs0, c.fcycle))
+
+
 @pytest.mark.parametrize("dim,ms", [(2, (64, 64)), (3, (16, 16, 16))])
 def test_cycle_coarse_program_new_operator(dim, ms):
     """Same plan, a second operator of the same kind: the coarse program's
